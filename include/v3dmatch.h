@@ -1,0 +1,177 @@
+/*
+ * v3dmatch.h -- C ABI of the MI355X SIFT putative-matching engine.
+ *
+ * The entry points replace the device side of V3D_CUDA::SIFT_Matching
+ * (reference: V3D/include/CUDA/v3d_cudamatching.h:15-67,
+ * V3D/src/CUDA/v3d_cudamatching.cpp:14-255) and its internal C ABI
+ * (V3D/include/CUDA/v3d_cudamatching.cuh:26-32).  Plain pointers and sizes
+ * only; every call returns a status code (V3DM_OK == 0) and the message of
+ * the last failure is available from v3dm_last_error().
+ *
+ * Descriptors are 128-D, row-major, one row per feature.  Float descriptors
+ * are what the reference receives; uint8 descriptors u are the same data as
+ * the floats u/512 (the reference matcher sees exactly those floats).
+ * Match lists are (left index, right index) int32 pairs in ascending left
+ * order, as produced by the reference mutual check
+ * (v3d_cudamatching.cpp:121-126).
+ *
+ * Threading: a context owns one HIP stream and its device buffers.  Calls on
+ * one context must not overlap; several contexts may coexist.
+ */
+#ifndef V3DMATCH_H
+#define V3DMATCH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define V3DM_OK 0
+#define V3DM_ERR_ARG (-1)         /* invalid argument */
+#define V3DM_ERR_STATE (-2)       /* call out of order (e.g. find before allocate) */
+#define V3DM_ERR_DEVICE (-3)      /* HIP runtime error or no usable device */
+#define V3DM_ERR_NOMEM (-4)       /* device or pinned-host allocation failed */
+#define V3DM_ERR_UNSUPPORTED (-5) /* request outside what the engine implements */
+#define V3DM_ERR_CAPACITY (-6)    /* caller's output buffer too small; *count holds the size needed */
+
+/* kernel selection for v3dm_set_kernel() */
+#define V3DM_KERNEL_AUTO 0  /* int8 MFMA when both sides are u8-representable, else plain FP32 */
+#define V3DM_KERNEL_PLAIN 1 /* reference design: FP32 score matrix in HBM + column/row passes */
+#define V3DM_KERNEL_I8 2    /* fused int8 MFMA matcher (bit-exact for u8 / u-over-512 inputs) */
+
+typedef struct v3dm_ctx v3dm_ctx;
+
+/* --- lifecycle ----------------------------------------------------------- */
+
+/* SIFT_Matching::SIFT_Matching() -- v3d_cudamatching.h:17-21.  Binds the
+ * context to HIP device `device` and creates its stream. */
+int v3dm_create(int device, v3dm_ctx **out);
+
+/* Frees everything the context holds (the reference leaves this to
+ * deallocate(); destroying also deallocates). */
+void v3dm_destroy(v3dm_ctx *ctx);
+
+/* Message of the last failed call on ctx ("" if none); ctx may be NULL for
+ * failures of v3dm_create. */
+const char *v3dm_last_error(const v3dm_ctx *ctx);
+
+/* enableRoundingWarning(bool) -- v3d_cudamatching.h:23-26 */
+int v3dm_set_rounding_warning(v3dm_ctx *ctx, int flag);
+
+/* enableReductionPath(bool) -- v3d_cudamatching.h:28-31.  Accepted and
+ * recorded; the match lists are those of the default (non-reduction) path. */
+int v3dm_set_reduction_path(v3dm_ctx *ctx, int flag);
+
+/* allocate(nMaxLeft, nMaxRight, useGuidedMatching) --
+ * v3d_cudamatching.h:33-37, v3d_cudamatching.cpp:14-49.  Maxima are rounded
+ * down to a multiple of 32 (warning on stderr when enabled). */
+int v3dm_allocate(v3dm_ctx *ctx, int nMaxLeft, int nMaxRight, int useGuidedMatching);
+
+/* deallocate() -- v3d_cudamatching.cpp:51-63 */
+int v3dm_deallocate(v3dm_ctx *ctx);
+
+/* --- descriptors ---------------------------------------------------------- */
+
+/* setLeftFeatures / setRightFeatures -- v3d_cudamatching.h:41-42,
+ * v3d_cudamatching.cpp:65-95.  n is rounded down to a multiple of 32
+ * (warning when enabled) and clamped to the allocated maximum; the host
+ * array is copied before the call returns. */
+int v3dm_set_left_f32(v3dm_ctx *ctx, int n, const float *desc);
+int v3dm_set_right_f32(v3dm_ctx *ctx, int n, const float *desc);
+
+/* Additive: the same calls for uint8 descriptors (equivalent to floats u/512). */
+int v3dm_set_left_u8(v3dm_ctx *ctx, int n, const uint8_t *desc);
+int v3dm_set_right_u8(v3dm_ctx *ctx, int n, const uint8_t *desc);
+
+/* Current (rounded, clamped) feature counts. */
+int v3dm_get_counts(const v3dm_ctx *ctx, int *nLeft, int *nRight);
+
+/* --- matching ------------------------------------------------------------- */
+
+/* findPutativeMatches(matches, minScore, minRatio) --
+ * v3d_cudamatching.h:44-46, v3d_cudamatching.cpp:97-127.  Ratio test off
+ * when minRatio < 0.  Writes min(count, capacity) pairs (2 int32 each);
+ * returns V3DM_ERR_CAPACITY when count > capacity. */
+int v3dm_find_putative_matches(v3dm_ctx *ctx, float minScore, float minRatio, int32_t *pairs, int capacity,
+                               int *count);
+
+/* runGuidedMatching(F, distanceThreshold, leftPositions, rightPositions,
+ * matches, minScore, minRatio) -- v3d_cudamatching.h:48-52,
+ * v3d_cudamatching.cpp:129-191, v3d_cudamatching.cu:259-293.  F: row-major
+ * 3x3 doubles; positions: interleaved (x, y) floats for the current left and
+ * right features.  Like the reference, the mask accumulates with the masks
+ * of earlier guided calls since the last findPutativeMatches.  Requires
+ * allocate(..., useGuidedMatching = 1). */
+int v3dm_run_guided_matching(v3dm_ctx *ctx, const double F[9], float distanceThreshold, const float *leftPos,
+                             const float *rightPos, float minScore, float minRatio, int32_t *pairs, int capacity,
+                             int *count);
+
+/* runGuidedHomographyMatching(...) -- v3d_cudamatching.h:54-58,
+ * v3d_cudamatching.cpp:193-255, v3d_cudamatching.cu:295-325. */
+int v3dm_run_guided_homography_matching(v3dm_ctx *ctx, const double H[9], float distanceThreshold,
+                                        const float *leftPos, const float *rightPos, float minScore,
+                                        float minRatio, int32_t *pairs, int capacity, int *count);
+
+/* Kernel selection (V3DM_KERNEL_*) and the kernel the last match call ran. */
+int v3dm_set_kernel(v3dm_ctx *ctx, int kernel);
+int v3dm_get_last_kernel(const v3dm_ctx *ctx, int *kernel);
+
+/* --- asynchronous form (device-resident inputs, no host round trip) ------- */
+
+/* The context's HIP stream (hipStream_t) for event timing. */
+int v3dm_get_stream(const v3dm_ctx *ctx, void **stream);
+
+/* Enqueue findPutativeMatches on the context stream and return at once; the
+ * per-left match table stays on the device until v3dm_fetch_matches(). */
+int v3dm_find_async(v3dm_ctx *ctx, float minScore, float minRatio);
+
+/* Wait for the last enqueued match and copy its list to the host. */
+int v3dm_fetch_matches(v3dm_ctx *ctx, int32_t *pairs, int capacity, int *count);
+
+int v3dm_synchronize(v3dm_ctx *ctx);
+
+/* --- kernel timing (HIP events recorded on the context stream) ------------ */
+
+/* When enabled, every match job records HIP events around its kernels
+ * (scan: the fused int8 scan or, on the plain path, the score-matrix
+ * kernel; select: finalize / selection passes).  Enabling resets the
+ * totals.  Up to 65536 jobs are kept. */
+int v3dm_profile_enable(v3dm_ctx *ctx, int enable);
+
+/* Waits for the stream, then returns the summed event times in ms of the
+ * jobs recorded since profiling was enabled, and their number. */
+int v3dm_profile_read(v3dm_ctx *ctx, double *scanMs, double *selectMs, int *jobs);
+
+/* --- synthetic SIFT-like descriptors (host) -------------------------------- */
+
+/* Generates one image pair: nL left descriptors, nR right descriptors of
+ * which min(floor(rho*nR), nL) are noisy copies (N(0, sigma) per component
+ * before renormalisation) of distinct left rows at random positions.  Any
+ * output pointer may be NULL.  rightSource[r] = source left row or -1. */
+int v3dm_synth_pair(uint64_t seed, int nL, int nR, double rho, double sigma, int nThreads, uint8_t *leftU8,
+                    uint8_t *rightU8, float *leftF32, float *rightF32, int32_t *rightSource);
+
+/* Seed of pair `pairIndex` of workload `config`: 0x5EED0000 + 256*config + pairIndex. */
+uint64_t v3dm_synth_seed(int config, int pairIndex);
+
+/* --- diagnostics ----------------------------------------------------------- */
+
+/* Number of HIP devices visible (0 without a GPU). */
+int v3dm_device_count(int *count);
+
+/* Runs one v_mfma_i32_32x32x32_i8 on device `device`, operands packed the way
+ * the matcher packs descriptors: row i of A holds (i, 32) in its first two
+ * bytes, column j of B holds (1, j), all other bytes 0, so D[i][j] = i + 32 j.
+ * Writes the 64 lanes x 16 accumulator registers (lane-major, 1024 int32) to
+ * out.  Tests use it to check the accumulator layout the matcher assumes:
+ * register v of lane l holds row 8*(v/4) + 4*(l/32) + v%4, column l%32. */
+int v3dm_mfma_layout_probe(int device, int32_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* V3DMATCH_H */

@@ -1,0 +1,233 @@
+"""GPU parity: every kernel's match list against the CPU oracle, bit for bit.
+
+All calls go through the C ABI (libv3dmatch.so) via the SIFTMatching mirror.
+u8 descriptors (and floats u/512) must give identical lists on the int8-MFMA
+kernel, the plain FP32 kernel and the oracle; general float descriptors give
+identical lists on the plain kernel and the oracle's sequential-FMA scores.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import v3dsfmtoolkit_amd as v
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+RULES = [(0.9, 0.8), (0.8, 0.8), (0.8, -1.0), (0.5, 0.95), (0.95, 0.6), (0.0, 0.0), (1.1, 0.8)]
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def m(gpu_available):
+    mm = v.SIFTMatching(0)
+    mm.enableRoundingWarning(False)
+    mm.allocate(16384, 16384, True)
+    yield mm
+    mm.close()
+
+
+def run(m, L, R, ms, mr, kernel=v.KERNEL_AUTO):
+    m.setKernel(kernel)
+    m.setLeftFeatures(len(L), L)
+    m.setRightFeatures(len(R), R)
+    return m.findPutativeMatches(ms, mr)
+
+
+def oracle_pairs(L, R, ms, mr, masks=None):
+    return O.table_to_pairs(O.find_u8(L, R, ms, mr, masks=masks)[2])
+
+
+def pair(cfg, idx, nL, nR, rho=0.4, sigma=0.05, f32=False):
+    return v.synth_pair(v.synth_seed(cfg, idx), nL, nR, rho, sigma, f32=f32)
+
+
+def test_mfma_accumulator_layout(gpu_available):
+    say("mfma layout probe")
+    out = v.mfma_layout_probe(0)
+    lane = np.arange(64)[:, None]
+    reg = np.arange(16)[None, :]
+    i = 8 * (reg // 4) + 4 * (lane // 32) + reg % 4
+    j = lane % 32
+    np.testing.assert_array_equal(out, i + 32 * j)
+
+
+@pytest.mark.parametrize("kernel", [v.KERNEL_I8, v.KERNEL_PLAIN])
+@pytest.mark.parametrize("rule", RULES)
+def test_c1_2048_matches_oracle(m, kernel, rule):
+    say("c1 2048x2048", kernel, rule)
+    d = pair(0, 0, 2048, 2048)
+    got = run(m, d["L"], d["R"], *rule, kernel=kernel)
+    assert m.lastKernel() == kernel
+    assert got == oracle_pairs(d["L"], d["R"], *rule)
+
+
+SHAPES = [(32, 32), (64, 96), (96, 1000), (1000, 96), (2080, 704), (256, 256), (288, 320), (4096, 128),
+          (128, 4096), (320, 2336)]
+
+
+@pytest.mark.parametrize("kernel", [v.KERNEL_I8, v.KERNEL_PLAIN])
+@pytest.mark.parametrize("shape", SHAPES)
+def test_shapes_match_oracle(m, kernel, shape):
+    nL, nR = shape
+    say("shape", shape, "kernel", kernel)
+    d = pair(0, 100 + nL % 97, nL, nR)
+    for rule in [(0.8, 0.8), (0.9, 0.8), (0.8, -1.0)]:
+        assert run(m, d["L"], d["R"], *rule, kernel=kernel) == oracle_pairs(d["L"], d["R"], *rule)
+
+
+@pytest.mark.parametrize("kernel", [v.KERNEL_I8, v.KERNEL_PLAIN])
+def test_rounding_and_clamp(m, kernel):
+    say("rounding/clamp", kernel)
+    d = pair(0, 3, 1000, 777)
+    m.setKernel(kernel)
+    m.setLeftFeatures(1000, d["L"])
+    m.setRightFeatures(777, d["R"])
+    assert m.counts() == (992, 768)  # v3d_cudamatching.cpp:68-77: rounded down to multiples of 32
+    assert m.findPutativeMatches(0.8, 0.8) == oracle_pairs(d["L"][:992], d["R"][:768], 0.8, 0.8)
+    small = v.SIFTMatching(0)
+    small.enableRoundingWarning(False)
+    small.allocate(500, 300)  # rounded to 480 x 288
+    small.setKernel(kernel)
+    small.setLeftFeatures(1000, d["L"])
+    small.setRightFeatures(777, d["R"])
+    assert small.counts() == (480, 288)
+    assert small.findPutativeMatches(0.8, 0.8) == oracle_pairs(d["L"][:480], d["R"][:288], 0.8, 0.8)
+    small.close()
+
+
+@pytest.mark.parametrize("kernel", [v.KERNEL_I8, v.KERNEL_PLAIN])
+def test_empty_sides(m, kernel):
+    say("empty", kernel)
+    d = pair(0, 4, 64, 64)
+    assert run(m, d["L"][:0], d["R"], 0.8, 0.8, kernel) == []
+    assert run(m, d["L"], d["R"][:0], 0.8, 0.8, kernel) == []
+    assert run(m, d["L"][:0], d["R"][:0], 0.8, 0.8, kernel) == []
+
+
+@pytest.mark.parametrize("kernel", [v.KERNEL_I8, v.KERNEL_PLAIN])
+def test_exact_duplicates_first_index_wins(m, kernel):
+    say("duplicates", kernel)
+    d = pair(0, 5, 512, 640)
+    L, R = d["L"].copy(), d["R"].copy()
+    L[300] = L[12]
+    L[301] = L[12]
+    R[100] = R[7]
+    R[500:520] = R[200:220]
+    src = d["src"]
+    k = int(np.flatnonzero(src >= 0)[0])
+    R[k + 1 if k + 1 < 640 else k - 1] = R[k]  # duplicated true match: ratio 1, rejected
+    for rule in [(0.8, 0.8), (0.8, -1.0), (0.0, 0.0), (0.5, 0.95)]:
+        assert run(m, L, R, *rule, kernel=kernel) == oracle_pairs(L, R, *rule)
+
+
+def test_identical_sets_worst_case_survivors(m):
+    # every score of the L = R diagonal survives; many near-duplicates
+    say("identity")
+    d = pair(0, 6, 1024, 64)
+    L = d["L"]
+    for rule in [(0.8, 0.8), (0.3, 0.99)]:
+        assert run(m, L, L, *rule, kernel=v.KERNEL_I8) == oracle_pairs(L, L, *rule)
+
+
+def test_float_inputs_dispatch(m):
+    say("float dispatch")
+    d = pair(0, 8, 700, 900, f32=True)
+    Lq = d["L"].astype(np.float32) / np.float32(512)
+    Rq = d["R"].astype(np.float32) / np.float32(512)
+    got = run(m, Lq, Rq, 0.8, 0.8, v.KERNEL_AUTO)
+    assert m.lastKernel() == v.KERNEL_I8
+    assert got == oracle_pairs(d["L"][:672], d["R"][:896], 0.8, 0.8)
+    # unquantised floats: plain kernel, bit-exact with the oracle's sequential-FMA scores
+    Lf, Rf = d["Lf"][:672], d["Rf"][:896]
+    for rule in [(0.8, 0.8), (0.9, 0.8), (0.8, -1.0)]:
+        got = run(m, Lf, Rf, *rule, v.KERNEL_AUTO)
+        assert m.lastKernel() == v.KERNEL_PLAIN
+        assert got == O.table_to_pairs(O.find_f32(Lf, Rf, *rule)[2])
+    m.setKernel(v.KERNEL_I8)
+    with pytest.raises(v.V3DMError):
+        m.findPutativeMatches(0.8, 0.8)
+    m.setKernel(v.KERNEL_AUTO)
+
+
+def _positions(d, nL, nR, seed):
+    """Left positions uniform in [0, 1000)^2; a right copy sits near the
+    horizontal epipolar line of its source (pure x-translation)."""
+    rng = np.random.default_rng(seed)
+    lpos = (rng.random((nL, 2)) * 1000).astype(np.float32)
+    rpos = (rng.random((nR, 2)) * 1000).astype(np.float32)
+    src = d["src"][:nR]
+    cp = src >= 0
+    rpos[cp, 1] = lpos[src[cp], 1] + rng.normal(0, 2.0, cp.sum()).astype(np.float32)
+    rpos[cp, 0] = lpos[src[cp], 0] + 30 + rng.normal(0, 2.0, cp.sum()).astype(np.float32)
+    return lpos, rpos
+
+
+@pytest.mark.parametrize("kernel", [v.KERNEL_I8, v.KERNEL_PLAIN])
+def test_guided_epipolar_and_homography(m, kernel):
+    say("guided", kernel)
+    nL, nR = 1024, 1280
+    d = pair(0, 9, nL, nR)
+    L, R = d["L"], d["R"]
+    lpos, rpos = _positions(d, nL, nR, 1)
+    F = np.array([[0, 0, 0], [0, 0, -1], [0, 1, 0]], np.float64)  # y-disparity Sampson test
+    H = np.array([[1, 0, 30], [0, 1, 0], [0, 0, 1]], np.float64)  # shift by +30 in x
+    m.setKernel(kernel)
+    m.setLeftFeatures(nL, L)
+    m.setRightFeatures(nR, R)
+    for rule in [(0.8, 0.8), (0.9, 0.8), (0.8, -1.0)]:
+        base = m.findPutativeMatches(*rule)
+        assert base == oracle_pairs(L, R, *rule)
+        g1 = m.runGuidedMatching(F, 3.0, lpos, rpos, *rule)
+        assert g1 == oracle_pairs(L, R, *rule, masks=[(1, F, 3.0, lpos, rpos)])
+        # a second guided call accumulates with the first (the score buffer is masked in place)
+        g2 = m.runGuidedHomographyMatching(H, 5.0, lpos, rpos, *rule)
+        assert g2 == oracle_pairs(L, R, *rule, masks=[(1, F, 3.0, lpos, rpos), (2, H, 5.0, lpos, rpos)])
+        # findPutativeMatches recomputes the scores: masks are gone
+        assert m.findPutativeMatches(*rule) == base
+
+
+def test_guided_requires_allocation_flag(gpu_available):
+    say("guided flag")
+    mm = v.SIFTMatching(0)
+    mm.allocate(256, 256)
+    d = pair(0, 10, 64, 64)
+    mm.setLeftFeatures(64, d["L"])
+    mm.setRightFeatures(64, d["R"])
+    mm.findPutativeMatches()
+    with pytest.raises(v.V3DMError):
+        mm.runGuidedMatching(np.eye(3), 1.0, np.zeros((64, 2)), np.zeros((64, 2)))
+    mm.close()
+
+
+def test_async_repeat_is_stable(m):
+    say("async repeat")
+    d = pair(0, 11, 3000, 2500)
+    m.setKernel(v.KERNEL_I8)
+    m.setLeftFeatures(3000, d["L"])
+    m.setRightFeatures(2500, d["R"])
+    want = oracle_pairs(d["L"][:2976], d["R"][:2496], 0.8, 0.8)
+    for _ in range(50):
+        m.findAsync(0.8, 0.8)
+    assert m.fetchMatches() == want
+    assert m.findPutativeMatches(0.8, 0.8) == want
+
+
+def test_c2_8192_i8_equals_plain_and_oracle(m):
+    say("c2 8192x8192")
+    d = pair(1, 0, 8192, 8192)
+    L, R = d["L"], d["R"]
+    i8 = run(m, L, R, 0.8, 0.8, v.KERNEL_I8)
+    plain = run(m, L, R, 0.8, 0.8, v.KERNEL_PLAIN)
+    assert i8 == plain
+    c, r, tab, _ = O.find_u8_blocked(L, R, 0.8, 0.8)
+    assert i8 == O.table_to_pairs(tab)
+    # size-independent property: every match of a planted copy points at its source
+    src = d["src"]
+    correct = sum(1 for k, l in i8 if src[l] == k)
+    assert correct >= 0.95 * len(i8) and len(i8) > 0.3 * 8192

@@ -1,0 +1,151 @@
+"""ctypes binding of libv3dmatch.so (the C ABI in include/v3dmatch.h).
+
+The shared library is built in-tree by ``__graft_entry__.build()`` (or
+``make -C v3dsfmtoolkit_amd/csrc``).  Loading fails loudly when it is
+missing: there is no fallback implementation.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libv3dmatch.so")
+HEADER_PATH = os.path.join(os.path.dirname(HERE), "include", "v3dmatch.h")
+
+V3DM_OK = 0
+V3DM_ERR_ARG = -1
+V3DM_ERR_STATE = -2
+V3DM_ERR_DEVICE = -3
+V3DM_ERR_NOMEM = -4
+V3DM_ERR_UNSUPPORTED = -5
+V3DM_ERR_CAPACITY = -6
+
+KERNEL_AUTO = 0
+KERNEL_PLAIN = 1
+KERNEL_I8 = 2
+
+_c_int_p = ctypes.POINTER(ctypes.c_int)
+_i32_p = ctypes.POINTER(ctypes.c_int32)
+_u8_p = ctypes.POINTER(ctypes.c_uint8)
+_f32_p = ctypes.POINTER(ctypes.c_float)
+_f64_p = ctypes.POINTER(ctypes.c_double)
+_ctx_p = ctypes.c_void_p
+
+_SIGNATURES = {
+    "v3dm_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    "v3dm_destroy": (None, [_ctx_p]),
+    "v3dm_last_error": (ctypes.c_char_p, [_ctx_p]),
+    "v3dm_set_rounding_warning": (ctypes.c_int, [_ctx_p, ctypes.c_int]),
+    "v3dm_set_reduction_path": (ctypes.c_int, [_ctx_p, ctypes.c_int]),
+    "v3dm_allocate": (ctypes.c_int, [_ctx_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "v3dm_deallocate": (ctypes.c_int, [_ctx_p]),
+    "v3dm_set_left_f32": (ctypes.c_int, [_ctx_p, ctypes.c_int, _f32_p]),
+    "v3dm_set_right_f32": (ctypes.c_int, [_ctx_p, ctypes.c_int, _f32_p]),
+    "v3dm_set_left_u8": (ctypes.c_int, [_ctx_p, ctypes.c_int, _u8_p]),
+    "v3dm_set_right_u8": (ctypes.c_int, [_ctx_p, ctypes.c_int, _u8_p]),
+    "v3dm_get_counts": (ctypes.c_int, [_ctx_p, _c_int_p, _c_int_p]),
+    "v3dm_find_putative_matches": (ctypes.c_int, [_ctx_p, ctypes.c_float, ctypes.c_float, _i32_p, ctypes.c_int, _c_int_p]),
+    "v3dm_run_guided_matching": (ctypes.c_int, [_ctx_p, _f64_p, ctypes.c_float, _f32_p, _f32_p, ctypes.c_float,
+                                                ctypes.c_float, _i32_p, ctypes.c_int, _c_int_p]),
+    "v3dm_run_guided_homography_matching": (ctypes.c_int, [_ctx_p, _f64_p, ctypes.c_float, _f32_p, _f32_p,
+                                                           ctypes.c_float, ctypes.c_float, _i32_p, ctypes.c_int,
+                                                           _c_int_p]),
+    "v3dm_set_kernel": (ctypes.c_int, [_ctx_p, ctypes.c_int]),
+    "v3dm_get_last_kernel": (ctypes.c_int, [_ctx_p, _c_int_p]),
+    "v3dm_get_stream": (ctypes.c_int, [_ctx_p, ctypes.POINTER(ctypes.c_void_p)]),
+    "v3dm_find_async": (ctypes.c_int, [_ctx_p, ctypes.c_float, ctypes.c_float]),
+    "v3dm_fetch_matches": (ctypes.c_int, [_ctx_p, _i32_p, ctypes.c_int, _c_int_p]),
+    "v3dm_synchronize": (ctypes.c_int, [_ctx_p]),
+    "v3dm_profile_enable": (ctypes.c_int, [_ctx_p, ctypes.c_int]),
+    "v3dm_profile_read": (ctypes.c_int, [_ctx_p, _f64_p, _f64_p, _c_int_p]),
+    "v3dm_synth_pair": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_int, _u8_p, _u8_p, _f32_p, _f32_p, _i32_p]),
+    "v3dm_synth_seed": (ctypes.c_uint64, [ctypes.c_int, ctypes.c_int]),
+    "v3dm_device_count": (ctypes.c_int, [_c_int_p]),
+    "v3dm_mfma_layout_probe": (ctypes.c_int, [ctypes.c_int, _i32_p]),
+}
+
+_lib = None
+
+
+def header_symbols(path: str = HEADER_PATH) -> list[str]:
+    """Names of every function declared in include/v3dmatch.h."""
+    text = open(path).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(v3dm_[a-z0-9_]+)\s*\(", text)))
+
+
+def lib() -> ctypes.CDLL:
+    """Load libv3dmatch.so once; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+                "or `make -C v3dsfmtoolkit_amd/csrc`")
+        handle = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def ptr(a: np.ndarray, ctype):
+    if a is None:
+        return None
+    return a.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+class V3DMError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"v3dmatch error {code}: {msg}")
+        self.code = code
+
+
+def check(code: int, ctx=None) -> None:
+    if code != V3DM_OK:
+        raise V3DMError(code, lib().v3dm_last_error(ctx).decode(errors="replace"))
+
+
+def device_count() -> int:
+    n = ctypes.c_int(0)
+    check(lib().v3dm_device_count(ctypes.byref(n)))
+    return n.value
+
+
+def synth_seed(config: int, pair_index: int) -> int:
+    return int(lib().v3dm_synth_seed(config, pair_index))
+
+
+def synth_pair(seed: int, n_left: int, n_right: int, rho: float = 0.4, sigma: float = 0.05, *,
+               u8: bool = True, f32: bool = False, threads: int = 0):
+    """Synthetic SIFT-like pair.  Returns a dict with 'L', 'R' (uint8),
+    'Lf', 'Rf' (float32, unquantised) as requested, and 'src' (right -> left
+    source row or -1)."""
+    out = {}
+    L = np.empty((n_left, 128), np.uint8) if u8 else None
+    R = np.empty((n_right, 128), np.uint8) if u8 else None
+    Lf = np.empty((n_left, 128), np.float32) if f32 else None
+    Rf = np.empty((n_right, 128), np.float32) if f32 else None
+    src = np.empty(n_right, np.int32)
+    check(lib().v3dm_synth_pair(seed, n_left, n_right, rho, sigma, threads, ptr(L, ctypes.c_uint8),
+                                ptr(R, ctypes.c_uint8), ptr(Lf, ctypes.c_float), ptr(Rf, ctypes.c_float),
+                                ptr(src, ctypes.c_int32)))
+    if u8:
+        out["L"], out["R"] = L, R
+    if f32:
+        out["Lf"], out["Rf"] = Lf, Rf
+    out["src"] = src
+    return out
+
+
+def mfma_layout_probe(device: int = 0) -> np.ndarray:
+    out = np.zeros((64, 16), np.int32)
+    check(lib().v3dm_mfma_layout_probe(device, ptr(out, ctypes.c_int32)))
+    return out

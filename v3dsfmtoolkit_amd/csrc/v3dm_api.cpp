@@ -1,0 +1,820 @@
+// v3dm_api.cpp -- the C ABI declared in include/v3dmatch.h.
+//
+// Host side of V3D_CUDA::SIFT_Matching (V3D/src/CUDA/v3d_cudamatching.cpp):
+// buffer ownership, the multiple-of-32 rounding and clamping of feature
+// counts, the stateful guided-matching contract, kernel dispatch and the
+// mutual-match list in ascending left order.
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/v3dmatch.h"
+#include "v3dm_internal.h"
+
+namespace v3dm {
+hipError_t launch_i8_scan(const ScanArgs &a, hipStream_t st);
+hipError_t launch_i8_finalize(const FinalizeArgs &a, hipStream_t st);
+hipError_t launch_clear_keys(uint64_t *keys, int64_t n, uint32_t *epoch, uint32_t *done, hipStream_t st);
+hipError_t launch_mfma_probe(int32_t *out, hipStream_t st);
+int scan_tile_m();
+int scan_tile_n();
+hipError_t launch_sgemm_seq(int nL, int nR, const float *Lf, const float *Rf, float *S, hipStream_t st);
+hipError_t launch_mask_plain(int nL, int nR, float *S, const Mask *masks, int nMasks, hipStream_t st);
+hipError_t launch_fill_i32(int n, int32_t value, int32_t *out, hipStream_t st);
+size_t plain_part_bytes(int nL, int nR, int *nChunksOut, int *rowsPerOut);
+hipError_t launch_plain_select(int nL, int nR, const float *S, void *part, float minScore, float minRatio,
+                               int32_t *colMax, int32_t *rowMax, int32_t *out, hipStream_t st);
+hipError_t launch_prep_u8(const uint8_t *src, int n, int nPad, int8_t *dst, int32_t *sum, hipStream_t st);
+hipError_t launch_prep_f32(const float *src, int n, int nPad, int8_t *dst, int32_t *sum, int *flag,
+                           hipStream_t st);
+hipError_t launch_u8_to_f32(const uint8_t *src, int64_t count, float *dst, hipStream_t st);
+} // namespace v3dm
+
+using namespace v3dm;
+
+namespace {
+
+thread_local std::string g_createError;
+
+int roundUp(int n, int m) { return (n + m - 1) / m * m; }
+
+// Survivor threshold in exact integer score units (score = int * 2^-18); see
+// the proof at the top of k_i8.hip.
+int32_t survivorThreshold(float minScore, float minRatio)
+{
+   double t;
+   if (minRatio < 0)
+      t = (double)minScore;
+   else if (minRatio > 0)
+   {
+      double tr = 1.0 - (1.0 - (double)minScore) / ((double)minRatio * (1.0 - 1e-5));
+      t = std::min((double)minScore, tr);
+   }
+   else
+      return -1; // minRatio == 0 or NaN: keep every score
+   if (!(t == t)) return -1;
+   double x = std::floor(t * 262144.0);
+   if (x < -1.0) return -1;
+   if (x > 2147483000.0) return 2147483000;
+   return (int32_t)x;
+}
+
+struct Side
+{
+   int n = 0;
+   int nMax = 0;
+   bool quantized = false; // int8 pool holds these features exactly
+   bool isU8 = false;      // last upload was uint8
+   bool f32Valid = false;  // dF32 holds the float form
+   uint8_t *dRawU8 = nullptr;
+   float *dF32 = nullptr; // float form (upload buffer for float input)
+   int8_t *dI8 = nullptr;
+   int32_t *dSum = nullptr;
+   uint64_t *dTop1 = nullptr, *dTop2 = nullptr;
+   float *dPos = nullptr; // kMaxMasks position sets (guided)
+};
+
+} // namespace
+
+struct v3dm_ctx
+{
+   int device = 0;
+   hipStream_t stream = nullptr;
+   std::string err;
+   bool warnRounding = true;
+   bool useReduction = false;
+   int kernelSel = V3DM_KERNEL_AUTO;
+   int lastKernel = 0;
+   bool allocated = false;
+   bool guided = false;
+   Side L, R;
+
+   PairDesc *dPair = nullptr;
+   PairDesc hPairUploaded{};
+   bool pairValid = false;
+   uint32_t *dEpoch = nullptr; // [0] = epoch, [1] = done counter
+   uint32_t hostEpoch = 1;
+   int32_t *dMatch = nullptr;
+   int32_t *hMatch = nullptr; // pinned
+   int *dFlag = nullptr;
+   int *hFlag = nullptr;      // pinned
+
+   std::vector<Mask> masks; // active since the last find
+   Mask *dMasks = nullptr;
+
+   // plain path
+   float *dS = nullptr;
+   size_t sBytes = 0;
+   void *dPart = nullptr;
+   size_t partBytes = 0;
+   int32_t *dColMax = nullptr, *dRowMax = nullptr;
+   bool sValid = false;     // dS holds the scores of the current descriptors
+   int sMasksApplied = 0;   // masks already applied to dS
+   bool lastWasPlain = false;
+
+   bool pendingAsync = false;
+   int lastNL = 0;
+
+   // kernel timing: three events per job (before scan, after scan, after select)
+   bool profile = false;
+   int profJobs = 0;
+   std::vector<hipEvent_t> events;
+};
+
+namespace {
+
+int fail(v3dm_ctx *c, int code, const char *fmt, ...)
+{
+   char buf[512];
+   va_list ap;
+   va_start(ap, fmt);
+   vsnprintf(buf, sizeof(buf), fmt, ap);
+   va_end(ap);
+   if (c) c->err = buf;
+   else g_createError = buf;
+   return code;
+}
+
+#define HIPCHK(c, call)                                                                                  \
+   do                                                                                                    \
+   {                                                                                                     \
+      hipError_t e_ = (call);                                                                            \
+      if (e_ != hipSuccess)                                                                              \
+         return fail((c), e_ == hipErrorOutOfMemory ? V3DM_ERR_NOMEM : V3DM_ERR_DEVICE, "%s: %s (%s:%d)", \
+                     #call, hipGetErrorString(e_), __FILE__, __LINE__);                                  \
+   } while (0)
+
+template <class T> void dfree(T *&p)
+{
+   if (p) (void)hipFree((void *)p);
+   p = nullptr;
+}
+
+void freeSide(Side &s)
+{
+   dfree(s.dRawU8);
+   dfree(s.dF32);
+   dfree(s.dI8);
+   dfree(s.dSum);
+   dfree(s.dTop1);
+   dfree(s.dTop2);
+   dfree(s.dPos);
+   s = Side();
+}
+
+constexpr int kMaxProfiledJobs = 65536;
+
+// Records event `which` (0, 1, 2) of the current profiled job.
+int profMark(v3dm_ctx *c, int which)
+{
+   if (!c->profile || c->profJobs >= kMaxProfiledJobs) return V3DM_OK;
+   const size_t idx = (size_t)c->profJobs * 3 + which;
+   while (c->events.size() <= idx)
+   {
+      hipEvent_t e;
+      HIPCHK(c, hipEventCreate(&e));
+      c->events.push_back(e);
+   }
+   HIPCHK(c, hipEventRecord(c->events[idx], c->stream));
+   if (which == 2) c->profJobs++;
+   return V3DM_OK;
+}
+
+void freeAll(v3dm_ctx *c)
+{
+   freeSide(c->L);
+   freeSide(c->R);
+   dfree(c->dPair);
+   dfree(c->dEpoch);
+   dfree(c->dMatch);
+   dfree(c->dFlag);
+   dfree(c->dMasks);
+   dfree(c->dS);
+   dfree(c->dPart);
+   dfree(c->dColMax);
+   dfree(c->dRowMax);
+   if (c->hMatch) (void)hipHostFree(c->hMatch);
+   if (c->hFlag) (void)hipHostFree(c->hFlag);
+   c->hMatch = nullptr;
+   c->hFlag = nullptr;
+   c->sBytes = c->partBytes = 0;
+   c->allocated = false;
+   c->pairValid = false;
+   c->sValid = false;
+   c->masks.clear();
+}
+
+int allocSide(v3dm_ctx *c, Side &s, int nMax, bool guided)
+{
+   s.nMax = nMax;
+   const int nPad = std::max(roundUp(nMax, kRowPad), kRowPad);
+   HIPCHK(c, hipMalloc((void **)&s.dI8, (size_t)nPad * kDim));
+   HIPCHK(c, hipMalloc((void **)&s.dSum, sizeof(int32_t) * nPad));
+   HIPCHK(c, hipMalloc((void **)&s.dTop1, sizeof(uint64_t) * nPad));
+   HIPCHK(c, hipMalloc((void **)&s.dTop2, sizeof(uint64_t) * nPad));
+   if (guided) HIPCHK(c, hipMalloc((void **)&s.dPos, sizeof(float) * 2 * (size_t)std::max(nMax, 1) * kMaxMasks));
+   return V3DM_OK;
+}
+
+int clearKeys(v3dm_ctx *c)
+{
+   const int nPadL = std::max(roundUp(c->L.nMax, kRowPad), kRowPad);
+   const int nPadR = std::max(roundUp(c->R.nMax, kRowPad), kRowPad);
+   HIPCHK(c, launch_clear_keys(c->L.dTop1, nPadL, c->dEpoch, c->dEpoch + 1, c->stream));
+   HIPCHK(c, launch_clear_keys(c->L.dTop2, nPadL, c->dEpoch, c->dEpoch + 1, c->stream));
+   HIPCHK(c, launch_clear_keys(c->R.dTop1, nPadR, c->dEpoch, c->dEpoch + 1, c->stream));
+   HIPCHK(c, launch_clear_keys(c->R.dTop2, nPadR, c->dEpoch, c->dEpoch + 1, c->stream));
+   c->hostEpoch = 1;
+   return V3DM_OK;
+}
+
+int roundFeatures(v3dm_ctx *c, int n, const char *who, int nMax)
+{
+   // v3d_cudamatching.cpp:68-77 / 84-93
+   if ((n % 32) != 0)
+   {
+      if (c->warnRounding)
+      {
+         fprintf(stderr, "SIFT_Matching::%s() warning: nFeatures should be a multiple of 32\n", who);
+         fprintf(stderr, "Rounding nFeatures down.\n");
+      }
+      n = n - (n & 31);
+   }
+   return std::min(n, nMax);
+}
+
+int syncStream(v3dm_ctx *c)
+{
+   HIPCHK(c, hipStreamSynchronize(c->stream));
+   return V3DM_OK;
+}
+
+int setFeatures(v3dm_ctx *c, Side &s, int n, const void *desc, bool u8, const char *who)
+{
+   if (!c) return V3DM_ERR_ARG;
+   if (!c->allocated) return fail(c, V3DM_ERR_STATE, "%s: allocate() has not been called", who);
+   if (n < 0) return fail(c, V3DM_ERR_ARG, "%s: negative feature count %d", who, n);
+   n = roundFeatures(c, n, who, s.nMax);
+   if (n > 0 && !desc) return fail(c, V3DM_ERR_ARG, "%s: descriptor pointer is NULL", who);
+   HIPCHK(c, hipSetDevice(c->device));
+   if (c->pendingAsync)
+   {
+      int rc = syncStream(c);
+      if (rc) return rc;
+      c->pendingAsync = false;
+   }
+   const int nPad = std::max(roundUp(std::max(n, 1), kRowPad), kRowPad);
+   s.n = n;
+   s.isU8 = u8;
+   if (u8)
+   {
+      if (!s.dRawU8) HIPCHK(c, hipMalloc((void **)&s.dRawU8, (size_t)std::max(s.nMax, 1) * kDim));
+      if (n > 0) HIPCHK(c, hipMemcpyAsync(s.dRawU8, desc, (size_t)n * kDim, hipMemcpyHostToDevice, c->stream));
+      HIPCHK(c, launch_prep_u8(s.dRawU8, n, nPad, s.dI8, s.dSum, c->stream));
+      s.quantized = true;
+      s.f32Valid = false;
+      int rc = syncStream(c);
+      if (rc) return rc;
+   }
+   else
+   {
+      if (!s.dF32) HIPCHK(c, hipMalloc((void **)&s.dF32, sizeof(float) * (size_t)std::max(s.nMax, 1) * kDim));
+      if (n > 0)
+         HIPCHK(c, hipMemcpyAsync(s.dF32, desc, sizeof(float) * (size_t)n * kDim, hipMemcpyHostToDevice, c->stream));
+      *c->hFlag = 0;
+      HIPCHK(c, hipMemsetAsync(c->dFlag, 0, sizeof(int), c->stream));
+      HIPCHK(c, launch_prep_f32(s.dF32, n, nPad, s.dI8, s.dSum, c->dFlag, c->stream));
+      HIPCHK(c, hipMemcpyAsync(c->hFlag, c->dFlag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      int rc = syncStream(c);
+      if (rc) return rc;
+      s.quantized = (*c->hFlag == 0);
+      s.f32Valid = true;
+   }
+   c->sValid = false;
+   return V3DM_OK;
+}
+
+int ensureF32(v3dm_ctx *c, Side &s)
+{
+   if (s.f32Valid) return V3DM_OK;
+   if (!s.dF32) HIPCHK(c, hipMalloc((void **)&s.dF32, sizeof(float) * (size_t)std::max(s.nMax, 1) * kDim));
+   HIPCHK(c, launch_u8_to_f32(s.dRawU8, (int64_t)s.n * kDim, s.dF32, c->stream));
+   s.f32Valid = true;
+   return V3DM_OK;
+}
+
+int uploadPair(v3dm_ctx *c)
+{
+   PairDesc pd{};
+   pd.offL = 0;
+   pd.offR = 0;
+   pd.slotL = 0;
+   pd.slotR = 0;
+   pd.nL = c->L.n;
+   pd.nR = c->R.n;
+   pd.tilesM = (roundUp(pd.nL, kRowPad) + scan_tile_m() - 1) / scan_tile_m();
+   pd.tilesN = (roundUp(pd.nR, kRowPad) + scan_tile_n() - 1) / scan_tile_n();
+   pd.tileBase = 0;
+   pd.finBase = 0;
+   if (c->pairValid && memcmp(&pd, &c->hPairUploaded, sizeof(pd)) == 0) return V3DM_OK;
+   if (c->pendingAsync)
+   {
+      int rc = syncStream(c);
+      if (rc) return rc;
+      c->pendingAsync = false;
+   }
+   HIPCHK(c, hipMemcpyAsync(c->dPair, &pd, sizeof(pd), hipMemcpyHostToDevice, c->stream));
+   int rc = syncStream(c);
+   if (rc) return rc;
+   c->hPairUploaded = pd;
+   c->pairValid = true;
+   return V3DM_OK;
+}
+
+int uploadMasks(v3dm_ctx *c)
+{
+   if (c->masks.empty()) return V3DM_OK;
+   HIPCHK(c, hipMemcpyAsync(c->dMasks, c->masks.data(), sizeof(Mask) * c->masks.size(), hipMemcpyHostToDevice,
+                            c->stream));
+   return syncStream(c);
+}
+
+int chooseKernel(v3dm_ctx *c, int *kernel)
+{
+   const bool q = c->L.quantized && c->R.quantized;
+   int k = c->kernelSel;
+   if (k == V3DM_KERNEL_AUTO) k = q ? V3DM_KERNEL_I8 : V3DM_KERNEL_PLAIN;
+   if (k == V3DM_KERNEL_I8 && !q)
+      return fail(c, V3DM_ERR_UNSUPPORTED,
+                  "int8 kernel requested but the descriptors are not all of the form u/512, u in [0,255]");
+   *kernel = k;
+   return V3DM_OK;
+}
+
+// Enqueue the i8 job for the current pair (masks already uploaded).
+int enqueueI8(v3dm_ctx *c, float minScore, float minRatio)
+{
+   int rc = uploadPair(c);
+   if (rc) return rc;
+   if (c->hostEpoch >= kEpochLimit)
+   {
+      rc = clearKeys(c);
+      if (rc) return rc;
+   }
+   const PairDesc &pd = c->hPairUploaded;
+   ScanArgs a{};
+   a.L = c->L.dI8;
+   a.Lsum = c->L.dSum;
+   a.R = c->R.dI8;
+   a.Rsum = c->R.dSum;
+   a.pairs = c->dPair;
+   a.nPairs = 1;
+   a.nTiles = (pd.nL > 0 && pd.nR > 0) ? pd.tilesM * pd.tilesN : 0;
+   a.thr = survivorThreshold(minScore, minRatio);
+   a.top1L = c->L.dTop1;
+   a.top2L = c->L.dTop2;
+   a.top1R = c->R.dTop1;
+   a.top2R = c->R.dTop2;
+   a.epoch = c->dEpoch;
+   a.masks = c->dMasks;
+   a.nMasks = (int)c->masks.size();
+   if ((rc = profMark(c, 0))) return rc;
+   HIPCHK(c, launch_i8_scan(a, c->stream));
+   if ((rc = profMark(c, 1))) return rc;
+   FinalizeArgs f{};
+   f.pairs = c->dPair;
+   f.nPairs = 1;
+   f.nBlocks = std::max(1, (pd.nL + 255) / 256);
+   f.top1L = c->L.dTop1;
+   f.top2L = c->L.dTop2;
+   f.top1R = c->R.dTop1;
+   f.top2R = c->R.dTop2;
+   f.epoch = c->dEpoch;
+   f.done = c->dEpoch + 1;
+   f.minScore = minScore;
+   f.minRatio = minRatio;
+   f.matchOfLeft = c->dMatch;
+   HIPCHK(c, launch_i8_finalize(f, c->stream));
+   if ((rc = profMark(c, 2))) return rc;
+   c->hostEpoch += 1;
+   return V3DM_OK;
+}
+
+// Plain path: scores (when stale), pending masks, selection passes.
+int enqueuePlain(v3dm_ctx *c, float minScore, float minRatio)
+{
+   const int nL = c->L.n, nR = c->R.n;
+   int rc = ensureF32(c, c->L);
+   if (rc) return rc;
+   rc = ensureF32(c, c->R);
+   if (rc) return rc;
+   const size_t sNeed = sizeof(float) * (size_t)std::max(nL, 1) * (size_t)std::max(nR, 1);
+   if (sNeed > c->sBytes)
+   {
+      if (c->pendingAsync)
+      {
+         rc = syncStream(c);
+         if (rc) return rc;
+         c->pendingAsync = false;
+      }
+      dfree(c->dS);
+      c->sBytes = 0;
+      c->sValid = false;
+      HIPCHK(c, hipMalloc((void **)&c->dS, sNeed));
+      c->sBytes = sNeed;
+   }
+   const size_t pNeed = plain_part_bytes(nL, nR, nullptr, nullptr);
+   if (pNeed > c->partBytes)
+   {
+      dfree(c->dPart);
+      c->partBytes = 0;
+      HIPCHK(c, hipMalloc(&c->dPart, pNeed));
+      c->partBytes = pNeed;
+   }
+   if ((rc = profMark(c, 0))) return rc;
+   if (!c->sValid)
+   {
+      HIPCHK(c, launch_sgemm_seq(nL, nR, c->L.dF32, c->R.dF32, c->dS, c->stream));
+      c->sValid = true;
+      c->sMasksApplied = 0;
+   }
+   if ((rc = profMark(c, 1))) return rc;
+   const int nm = (int)c->masks.size();
+   if (nm > c->sMasksApplied)
+   {
+      HIPCHK(c, launch_mask_plain(nL, nR, c->dS, c->dMasks + c->sMasksApplied, nm - c->sMasksApplied, c->stream));
+      c->sMasksApplied = nm;
+   }
+   HIPCHK(c, launch_plain_select(nL, nR, c->dS, c->dPart, minScore, minRatio, c->dColMax, c->dRowMax, c->dMatch,
+                                 c->stream));
+   return profMark(c, 2);
+}
+
+int runMatch(v3dm_ctx *c, float minScore, float minRatio, bool fromFind)
+{
+   if (!c->allocated) return fail(c, V3DM_ERR_STATE, "findPutativeMatches: allocate() has not been called");
+   HIPCHK(c, hipSetDevice(c->device));
+   int kernel = 0;
+   int rc = chooseKernel(c, &kernel);
+   if (rc) return rc;
+   if (fromFind)
+   {
+      // findPutativeMatches recomputes the score matrix: earlier guided masks
+      // no longer apply (v3d_cudamatching.cpp:104-105).
+      c->masks.clear();
+      c->sValid = false;
+   }
+   rc = kernel == V3DM_KERNEL_I8 ? enqueueI8(c, minScore, minRatio) : enqueuePlain(c, minScore, minRatio);
+   if (rc) return rc;
+   c->lastKernel = kernel;
+   c->lastNL = c->L.n;
+   c->pendingAsync = true;
+   return V3DM_OK;
+}
+
+int fetch(v3dm_ctx *c, int32_t *pairs, int capacity, int *count)
+{
+   const int nL = c->lastNL;
+   if (nL > 0)
+      HIPCHK(c, hipMemcpyAsync(c->hMatch, c->dMatch, sizeof(int32_t) * nL, hipMemcpyDeviceToHost, c->stream));
+   int rc = syncStream(c);
+   if (rc) return rc;
+   c->pendingAsync = false;
+   int n = 0;
+   for (int k = 0; k < nL; ++k)
+   {
+      const int l = c->hMatch[k];
+      if (l < 0) continue;
+      if (n < capacity && pairs)
+      {
+         pairs[2 * n] = k;
+         pairs[2 * n + 1] = l;
+      }
+      ++n;
+   }
+   if (count) *count = n;
+   if (n > capacity) return fail(c, V3DM_ERR_CAPACITY, "match list has %d pairs, capacity is %d", n, capacity);
+   return V3DM_OK;
+}
+
+int guidedCommon(v3dm_ctx *c, int kind, const double M[9], float distanceThreshold, const float *leftPos,
+                 const float *rightPos, float minScore, float minRatio, int32_t *pairs, int capacity, int *count)
+{
+   if (!c) return V3DM_ERR_ARG;
+   const char *who = kind == 1 ? "runGuidedMatching" : "runGuidedHomographyMatching";
+   if (!c->allocated) return fail(c, V3DM_ERR_STATE, "%s: allocate() has not been called", who);
+   if (!c->guided)
+      return fail(c, V3DM_ERR_STATE, "%s: allocate() was called without useGuidedMatching", who);
+   if (!M) return fail(c, V3DM_ERR_ARG, "%s: matrix pointer is NULL", who);
+   if ((c->L.n > 0 && !leftPos) || (c->R.n > 0 && !rightPos))
+      return fail(c, V3DM_ERR_ARG, "%s: position pointer is NULL", who);
+   if ((int)c->masks.size() >= kMaxMasks)
+      return fail(c, V3DM_ERR_UNSUPPORTED, "%s: more than %d guided calls since the last findPutativeMatches", who,
+                  kMaxMasks);
+   HIPCHK(c, hipSetDevice(c->device));
+   if (c->pendingAsync)
+   {
+      int rc = syncStream(c);
+      if (rc) return rc;
+      c->pendingAsync = false;
+   }
+   const int slot = (int)c->masks.size();
+   float *lp = c->L.dPos + (size_t)slot * 2 * std::max(c->L.nMax, 1);
+   float *rp = c->R.dPos + (size_t)slot * 2 * std::max(c->R.nMax, 1);
+   if (c->L.n > 0)
+      HIPCHK(c, hipMemcpyAsync(lp, leftPos, sizeof(float) * 2 * c->L.n, hipMemcpyHostToDevice, c->stream));
+   if (c->R.n > 0)
+      HIPCHK(c, hipMemcpyAsync(rp, rightPos, sizeof(float) * 2 * c->R.n, hipMemcpyHostToDevice, c->stream));
+   Mask mk{};
+   mk.kind = kind;
+   mk.sqrThr = distanceThreshold * distanceThreshold; // v3d_cudamatching.cu:414 (float)
+   for (int i = 0; i < 9; ++i) mk.m[i] = (float)M[i];  // v3d_cudamatching.cpp:161-164
+   mk.lpos = lp;
+   mk.rpos = rp;
+   c->masks.push_back(mk);
+   int rc = uploadMasks(c);
+   if (rc) return rc;
+   rc = runMatch(c, minScore, minRatio, false);
+   if (rc) return rc;
+   return fetch(c, pairs, capacity, count);
+}
+
+} // namespace
+
+// ============================================================================
+
+extern "C" {
+
+int v3dm_create(int device, v3dm_ctx **out)
+{
+   if (!out) return fail(nullptr, V3DM_ERR_ARG, "v3dm_create: out is NULL");
+   *out = nullptr;
+   int n = 0;
+   hipError_t e = hipGetDeviceCount(&n);
+   if (e != hipSuccess || n <= 0)
+      return fail(nullptr, V3DM_ERR_DEVICE, "v3dm_create: no HIP device available (%s)",
+                  e != hipSuccess ? hipGetErrorString(e) : "0 devices");
+   if (device < 0 || device >= n)
+      return fail(nullptr, V3DM_ERR_ARG, "v3dm_create: device %d out of range [0, %d)", device, n);
+   e = hipSetDevice(device);
+   if (e != hipSuccess) return fail(nullptr, V3DM_ERR_DEVICE, "hipSetDevice: %s", hipGetErrorString(e));
+   v3dm_ctx *c = new v3dm_ctx();
+   c->device = device;
+   e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+   if (e != hipSuccess)
+   {
+      delete c;
+      return fail(nullptr, V3DM_ERR_DEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
+   }
+   *out = c;
+   return V3DM_OK;
+}
+
+void v3dm_destroy(v3dm_ctx *c)
+{
+   if (!c) return;
+   (void)hipSetDevice(c->device);
+   if (c->stream) (void)hipStreamSynchronize(c->stream);
+   freeAll(c);
+   for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
+   if (c->stream) (void)hipStreamDestroy(c->stream);
+   delete c;
+}
+
+const char *v3dm_last_error(const v3dm_ctx *c) { return c ? c->err.c_str() : g_createError.c_str(); }
+
+int v3dm_set_rounding_warning(v3dm_ctx *c, int flag)
+{
+   if (!c) return V3DM_ERR_ARG;
+   c->warnRounding = flag != 0;
+   return V3DM_OK;
+}
+
+int v3dm_set_reduction_path(v3dm_ctx *c, int flag)
+{
+   if (!c) return V3DM_ERR_ARG;
+   c->useReduction = flag != 0;
+   return V3DM_OK;
+}
+
+int v3dm_allocate(v3dm_ctx *c, int nMaxLeft, int nMaxRight, int useGuided)
+{
+   if (!c) return V3DM_ERR_ARG;
+   if (nMaxLeft < 0 || nMaxRight < 0)
+      return fail(c, V3DM_ERR_ARG, "allocate: negative maximum (%d, %d)", nMaxLeft, nMaxRight);
+   // v3d_cudamatching.cpp:19-35
+   if (nMaxLeft % 32)
+   {
+      if (c->warnRounding)
+      {
+         fprintf(stderr, "SIFT_Matching::allocate() warning: nMaxLeftFeatures should be a multiple of 32\n");
+         fprintf(stderr, "Rounding nMaxLeftFeatures down.\n");
+      }
+      nMaxLeft -= nMaxLeft & 31;
+   }
+   if (nMaxRight % 32)
+   {
+      if (c->warnRounding)
+      {
+         fprintf(stderr, "SIFT_Matching::allocate() warning: nMaxRightFeatures should be a multiple of 32\n");
+         fprintf(stderr, "Rounding nMaxRightFeatures down.\n");
+      }
+      nMaxRight -= nMaxRight & 31;
+   }
+   if (nMaxLeft > kMaxLocalIndex || nMaxRight > kMaxLocalIndex)
+      return fail(c, V3DM_ERR_UNSUPPORTED, "allocate: at most %d features per image", kMaxLocalIndex);
+   HIPCHK(c, hipSetDevice(c->device));
+   if (c->allocated)
+   {
+      (void)hipStreamSynchronize(c->stream);
+      freeAll(c);
+   }
+   c->guided = useGuided != 0;
+   int rc = allocSide(c, c->L, nMaxLeft, c->guided);
+   if (!rc) rc = allocSide(c, c->R, nMaxRight, c->guided);
+   if (rc)
+   {
+      freeAll(c);
+      return rc;
+   }
+   const int nPadL = std::max(roundUp(nMaxLeft, kRowPad), kRowPad);
+   HIPCHK(c, hipMalloc((void **)&c->dPair, sizeof(PairDesc)));
+   HIPCHK(c, hipMalloc((void **)&c->dEpoch, 2 * sizeof(uint32_t)));
+   HIPCHK(c, hipMalloc((void **)&c->dMatch, sizeof(int32_t) * nPadL));
+   HIPCHK(c, hipMalloc((void **)&c->dFlag, sizeof(int)));
+   HIPCHK(c, hipMalloc((void **)&c->dMasks, sizeof(Mask) * kMaxMasks));
+   HIPCHK(c, hipMalloc((void **)&c->dColMax, sizeof(int32_t) * nPadL));
+   HIPCHK(c, hipMalloc((void **)&c->dRowMax, sizeof(int32_t) * std::max(roundUp(nMaxRight, kRowPad), kRowPad)));
+   HIPCHK(c, hipHostMalloc((void **)&c->hMatch, sizeof(int32_t) * nPadL, hipHostMallocDefault));
+   HIPCHK(c, hipHostMalloc((void **)&c->hFlag, sizeof(int), hipHostMallocDefault));
+   c->allocated = true;
+   int r2 = clearKeys(c);
+   if (r2) return r2;
+   return syncStream(c);
+}
+
+int v3dm_deallocate(v3dm_ctx *c)
+{
+   if (!c) return V3DM_ERR_ARG;
+   (void)hipSetDevice(c->device);
+   if (c->stream) (void)hipStreamSynchronize(c->stream);
+   freeAll(c);
+   return V3DM_OK;
+}
+
+int v3dm_set_left_f32(v3dm_ctx *c, int n, const float *d)
+{
+   return c ? setFeatures(c, c->L, n, d, false, "setLeftFeatures") : V3DM_ERR_ARG;
+}
+int v3dm_set_right_f32(v3dm_ctx *c, int n, const float *d)
+{
+   return c ? setFeatures(c, c->R, n, d, false, "setRightFeatures") : V3DM_ERR_ARG;
+}
+int v3dm_set_left_u8(v3dm_ctx *c, int n, const uint8_t *d)
+{
+   return c ? setFeatures(c, c->L, n, d, true, "setLeftFeatures") : V3DM_ERR_ARG;
+}
+int v3dm_set_right_u8(v3dm_ctx *c, int n, const uint8_t *d)
+{
+   return c ? setFeatures(c, c->R, n, d, true, "setRightFeatures") : V3DM_ERR_ARG;
+}
+
+int v3dm_get_counts(const v3dm_ctx *c, int *nLeft, int *nRight)
+{
+   if (!c) return V3DM_ERR_ARG;
+   if (nLeft) *nLeft = c->L.n;
+   if (nRight) *nRight = c->R.n;
+   return V3DM_OK;
+}
+
+int v3dm_find_putative_matches(v3dm_ctx *c, float minScore, float minRatio, int32_t *pairs, int capacity,
+                               int *count)
+{
+   if (!c) return V3DM_ERR_ARG;
+   if (capacity < 0) return fail(c, V3DM_ERR_ARG, "findPutativeMatches: negative capacity");
+   int rc = runMatch(c, minScore, minRatio, true);
+   if (rc) return rc;
+   return fetch(c, pairs, capacity, count);
+}
+
+int v3dm_run_guided_matching(v3dm_ctx *c, const double F[9], float thr, const float *lp, const float *rp,
+                             float minScore, float minRatio, int32_t *pairs, int capacity, int *count)
+{
+   return guidedCommon(c, 1, F, thr, lp, rp, minScore, minRatio, pairs, capacity, count);
+}
+
+int v3dm_run_guided_homography_matching(v3dm_ctx *c, const double H[9], float thr, const float *lp,
+                                        const float *rp, float minScore, float minRatio, int32_t *pairs,
+                                        int capacity, int *count)
+{
+   return guidedCommon(c, 2, H, thr, lp, rp, minScore, minRatio, pairs, capacity, count);
+}
+
+int v3dm_set_kernel(v3dm_ctx *c, int kernel)
+{
+   if (!c) return V3DM_ERR_ARG;
+   if (kernel < V3DM_KERNEL_AUTO || kernel > V3DM_KERNEL_I8)
+      return fail(c, V3DM_ERR_ARG, "set_kernel: unknown kernel %d", kernel);
+   c->kernelSel = kernel;
+   return V3DM_OK;
+}
+
+int v3dm_get_last_kernel(const v3dm_ctx *c, int *kernel)
+{
+   if (!c || !kernel) return V3DM_ERR_ARG;
+   *kernel = c->lastKernel;
+   return V3DM_OK;
+}
+
+int v3dm_get_stream(const v3dm_ctx *c, void **stream)
+{
+   if (!c || !stream) return V3DM_ERR_ARG;
+   *stream = (void *)c->stream;
+   return V3DM_OK;
+}
+
+int v3dm_find_async(v3dm_ctx *c, float minScore, float minRatio)
+{
+   if (!c) return V3DM_ERR_ARG;
+   return runMatch(c, minScore, minRatio, true);
+}
+
+int v3dm_fetch_matches(v3dm_ctx *c, int32_t *pairs, int capacity, int *count)
+{
+   if (!c) return V3DM_ERR_ARG;
+   if (capacity < 0) return fail(c, V3DM_ERR_ARG, "fetch_matches: negative capacity");
+   HIPCHK(c, hipSetDevice(c->device));
+   return fetch(c, pairs, capacity, count);
+}
+
+int v3dm_synchronize(v3dm_ctx *c)
+{
+   if (!c) return V3DM_ERR_ARG;
+   HIPCHK(c, hipSetDevice(c->device));
+   int rc = syncStream(c);
+   if (!rc) c->pendingAsync = false;
+   return rc;
+}
+
+int v3dm_profile_enable(v3dm_ctx *c, int enable)
+{
+   if (!c) return V3DM_ERR_ARG;
+   HIPCHK(c, hipSetDevice(c->device));
+   int rc = syncStream(c);
+   if (rc) return rc;
+   c->profile = enable != 0;
+   c->profJobs = 0;
+   return V3DM_OK;
+}
+
+int v3dm_profile_read(v3dm_ctx *c, double *scanMs, double *selectMs, int *jobs)
+{
+   if (!c) return V3DM_ERR_ARG;
+   HIPCHK(c, hipSetDevice(c->device));
+   int rc = syncStream(c);
+   if (rc) return rc;
+   double a = 0.0, b = 0.0;
+   for (int j = 0; j < c->profJobs; ++j)
+   {
+      float t01 = 0.f, t12 = 0.f;
+      HIPCHK(c, hipEventElapsedTime(&t01, c->events[3 * j], c->events[3 * j + 1]));
+      HIPCHK(c, hipEventElapsedTime(&t12, c->events[3 * j + 1], c->events[3 * j + 2]));
+      a += t01;
+      b += t12;
+   }
+   if (scanMs) *scanMs = a;
+   if (selectMs) *selectMs = b;
+   if (jobs) *jobs = c->profJobs;
+   return V3DM_OK;
+}
+
+int v3dm_device_count(int *count)
+{
+   if (!count) return V3DM_ERR_ARG;
+   int n = 0;
+   if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+   *count = n;
+   return V3DM_OK;
+}
+
+int v3dm_mfma_layout_probe(int device, int32_t *out)
+{
+   if (!out) return fail(nullptr, V3DM_ERR_ARG, "mfma_layout_probe: out is NULL");
+   int n = 0;
+   if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n)
+      return fail(nullptr, V3DM_ERR_DEVICE, "mfma_layout_probe: device %d not available", device);
+   hipError_t e = hipSetDevice(device);
+   int32_t *d = nullptr;
+   if (e == hipSuccess) e = hipMalloc((void **)&d, sizeof(int32_t) * 1024);
+   if (e == hipSuccess) e = launch_mfma_probe(d, nullptr);
+   if (e == hipSuccess) e = hipMemcpy(out, d, sizeof(int32_t) * 1024, hipMemcpyDeviceToHost);
+   if (d) (void)hipFree(d);
+   if (e != hipSuccess) return fail(nullptr, V3DM_ERR_DEVICE, "mfma_layout_probe: %s", hipGetErrorString(e));
+   return V3DM_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,78 @@
+// v3dm_internal.h -- types shared by the host API and the HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace v3dm {
+
+constexpr int kDim = 128;          // descriptor length
+constexpr int kRowPad = 64;        // descriptor pools are padded to multiples of 64 rows
+constexpr int kMaxMasks = 16;      // guided masks accumulated between two plain finds
+constexpr int kMaxLocalIndex = (1 << 21) - 1; // per-image feature limit of the packed keys
+constexpr uint32_t kEpochLimit = (1u << 20) - 2;
+
+// One image pair of a match job.  Descriptor rows live in pools shared by
+// all pairs (an image used by several pairs is stored once); the per-pair
+// top-2 state lives in slots [slotL, slotL + nL) and [slotR, slotR + nR).
+struct PairDesc
+{
+   int64_t offL, offR;   // first pool row of the left / right image
+   int64_t slotL, slotR; // first top-2 slot of this pair's left / right side
+   int nL, nR;           // feature counts
+   int tilesM, tilesN;   // scan tiles along left / right
+   int tileBase;         // index of this pair's first scan tile
+   int finBase;          // index of this pair's first finalize block
+};
+
+// Guided-matching mask, applied to candidate (left p, right q) positions.
+struct Mask
+{
+   int kind;           // 1 = epipolar (Sampson), 2 = homography transfer
+   float sqrThr;       // distanceThreshold^2 computed in float
+   float m[9];         // row-major 3x3, converted from double
+   const float *lpos;  // interleaved (x, y) per left feature, as given to this call
+   const float *rpos;  // interleaved (x, y) per right feature
+};
+
+// Packed top-2 key: [63:44] epoch | [43:21] score | [20:0] (kMaxLocalIndex - index).
+// A larger key is a larger score, ties broken towards the smaller index,
+// which is the reference's first-index rule.
+__host__ __device__ inline uint64_t pack_key(uint32_t epoch, uint32_t score, uint32_t index)
+{
+   return ((uint64_t)epoch << 44) | ((uint64_t)score << 21) | (uint64_t)(kMaxLocalIndex - index);
+}
+__host__ __device__ inline uint32_t key_epoch(uint64_t k) { return (uint32_t)(k >> 44); }
+__host__ __device__ inline uint32_t key_score(uint64_t k) { return (uint32_t)((k >> 21) & 0x7FFFFFu); }
+__host__ __device__ inline uint32_t key_index(uint64_t k) { return (uint32_t)(kMaxLocalIndex - (uint32_t)(k & 0x1FFFFFu)); }
+
+struct ScanArgs
+{
+   const int8_t *L;     // left pool, (u - 128) as int8, rows padded to kRowPad
+   const int32_t *Lsum; // 128 * sum(u) per left row (0 for pad rows)
+   const int8_t *R;
+   const int32_t *Rsum;
+   const PairDesc *pairs;
+   int nPairs;
+   int nTiles;
+   int32_t thr;                 // candidate survives iff exact integer score > thr
+   uint64_t *top1L, *top2L;     // per left slot
+   uint64_t *top1R, *top2R;     // per right slot
+   const uint32_t *epoch;       // device-resident epoch of the current job
+   const Mask *masks;           // guided masks (single-pair jobs only)
+   int nMasks;
+};
+
+struct FinalizeArgs
+{
+   const PairDesc *pairs;
+   int nPairs;
+   int nBlocks;
+   const uint64_t *top1L, *top2L, *top1R, *top2R;
+   uint32_t *epoch;
+   uint32_t *done;
+   float minScore, minRatio;
+   int32_t *matchOfLeft; // per left slot: matched right index or -1
+};
+
+} // namespace v3dm

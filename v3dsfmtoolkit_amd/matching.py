@@ -1,0 +1,169 @@
+"""Python mirror of V3D_CUDA::SIFT_Matching over the C ABI.
+
+Same method names, argument meaning and defaults as the reference class
+(V3D/include/CUDA/v3d_cudamatching.h:15-67): allocate / deallocate /
+setLeftFeatures / setRightFeatures / findPutativeMatches /
+runGuidedMatching / runGuidedHomographyMatching / enableRoundingWarning /
+enableReductionPath.  Match lists are lists of (left, right) index tuples
+in ascending left order.  Device errors raise V3DMError (the reference
+exits the process instead, v3d_cudamatching.cuh:9-24).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, ptr
+
+
+class SIFTMatching:
+    """One matcher instance bound to one GPU (one HIP stream)."""
+
+    def __init__(self, device: int = 0):
+        self._lib = _lib.lib()
+        h = ctypes.c_void_p()
+        check(self._lib.v3dm_create(device, ctypes.byref(h)))
+        self._ctx = h
+        self._cap = 0
+
+    # -- reference API ------------------------------------------------------
+    def enableRoundingWarning(self, flag: bool = True) -> None:
+        check(self._lib.v3dm_set_rounding_warning(self._ctx, int(flag)), self._ctx)
+
+    def enableReductionPath(self, flag: bool = True) -> None:
+        check(self._lib.v3dm_set_reduction_path(self._ctx, int(flag)), self._ctx)
+
+    def allocate(self, nMaxLeftFeatures: int, nMaxRightFeatures: int | bool | None = None,
+                 useGuidedMatching: bool = False) -> None:
+        # allocate(nMax, useGuided) and allocate(nMaxL, nMaxR, useGuided)
+        if nMaxRightFeatures is None or isinstance(nMaxRightFeatures, bool):
+            if isinstance(nMaxRightFeatures, bool):
+                useGuidedMatching = nMaxRightFeatures
+            nMaxRightFeatures = nMaxLeftFeatures
+        check(self._lib.v3dm_allocate(self._ctx, int(nMaxLeftFeatures), int(nMaxRightFeatures),
+                                      int(useGuidedMatching)), self._ctx)
+        self._cap = max(int(nMaxLeftFeatures), 1)
+
+    def deallocate(self) -> None:
+        check(self._lib.v3dm_deallocate(self._ctx), self._ctx)
+
+    def setLeftFeatures(self, nFeatures: int, descriptors: np.ndarray) -> None:
+        self._set(True, nFeatures, descriptors)
+
+    def setRightFeatures(self, nFeatures: int, descriptors: np.ndarray) -> None:
+        self._set(False, nFeatures, descriptors)
+
+    def findPutativeMatches(self, minScore: float = 0.9, minRatio: float = 0.8) -> list[tuple[int, int]]:
+        pairs, count = self._buffer()
+        check(self._lib.v3dm_find_putative_matches(self._ctx, minScore, minRatio, ptr(pairs, ctypes.c_int32),
+                                                   self._cap, ctypes.byref(count)), self._ctx)
+        return _as_list(pairs, count.value)
+
+    def runGuidedMatching(self, F, distanceThreshold: float, leftPositions, rightPositions,
+                          minScore: float = 0.9, minRatio: float = 0.8) -> list[tuple[int, int]]:
+        return self._guided(self._lib.v3dm_run_guided_matching, F, distanceThreshold, leftPositions,
+                            rightPositions, minScore, minRatio)
+
+    def runGuidedHomographyMatching(self, H, distanceThreshold: float, leftPositions, rightPositions,
+                                    minScore: float = 0.9, minRatio: float = 0.8) -> list[tuple[int, int]]:
+        return self._guided(self._lib.v3dm_run_guided_homography_matching, H, distanceThreshold, leftPositions,
+                            rightPositions, minScore, minRatio)
+
+    # -- additions ------------------------------------------------------------
+    def setKernel(self, kernel: int) -> None:
+        check(self._lib.v3dm_set_kernel(self._ctx, int(kernel)), self._ctx)
+
+    def lastKernel(self) -> int:
+        k = ctypes.c_int(0)
+        check(self._lib.v3dm_get_last_kernel(self._ctx, ctypes.byref(k)), self._ctx)
+        return k.value
+
+    def counts(self) -> tuple[int, int]:
+        a, b = ctypes.c_int(0), ctypes.c_int(0)
+        check(self._lib.v3dm_get_counts(self._ctx, ctypes.byref(a), ctypes.byref(b)), self._ctx)
+        return a.value, b.value
+
+    def stream(self) -> int:
+        s = ctypes.c_void_p()
+        check(self._lib.v3dm_get_stream(self._ctx, ctypes.byref(s)), self._ctx)
+        return s.value or 0
+
+    def findAsync(self, minScore: float = 0.9, minRatio: float = 0.8) -> None:
+        check(self._lib.v3dm_find_async(self._ctx, minScore, minRatio), self._ctx)
+
+    def fetchMatches(self) -> list[tuple[int, int]]:
+        pairs, count = self._buffer()
+        check(self._lib.v3dm_fetch_matches(self._ctx, ptr(pairs, ctypes.c_int32), self._cap, ctypes.byref(count)),
+              self._ctx)
+        return _as_list(pairs, count.value)
+
+    def matchTable(self, pairs: list[tuple[int, int]] | None = None) -> np.ndarray:
+        """Per-left table (right index or -1) of the given or last list."""
+        n_left, _ = self.counts()
+        out = np.full(n_left, -1, np.int32)
+        for k, l in (pairs if pairs is not None else self.fetchMatches()):
+            out[k] = l
+        return out
+
+    def profileEnable(self, enable: bool = True) -> None:
+        check(self._lib.v3dm_profile_enable(self._ctx, int(enable)), self._ctx)
+
+    def profileRead(self) -> tuple[float, float, int]:
+        """(scan ms, select ms, jobs) summed over the jobs since profileEnable."""
+        a, b, n = ctypes.c_double(0), ctypes.c_double(0), ctypes.c_int(0)
+        check(self._lib.v3dm_profile_read(self._ctx, ctypes.byref(a), ctypes.byref(b), ctypes.byref(n)), self._ctx)
+        return a.value, b.value, n.value
+
+    def synchronize(self) -> None:
+        check(self._lib.v3dm_synchronize(self._ctx), self._ctx)
+
+    def close(self) -> None:
+        if getattr(self, "_ctx", None):
+            self._lib.v3dm_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- helpers ---------------------------------------------------------------
+    def _buffer(self):
+        return np.empty((max(self._cap, 1), 2), np.int32), ctypes.c_int(0)
+
+    def _set(self, left: bool, n: int, desc: np.ndarray) -> None:
+        desc = np.ascontiguousarray(desc)
+        if desc.ndim != 2 or desc.shape[1] != 128:
+            raise ValueError("descriptors must be an (n, 128) array")
+        if int(n) > desc.shape[0]:
+            raise ValueError("nFeatures exceeds the number of descriptor rows")
+        if desc.dtype == np.uint8:
+            fn = self._lib.v3dm_set_left_u8 if left else self._lib.v3dm_set_right_u8
+            check(fn(self._ctx, int(n), ptr(desc, ctypes.c_uint8)), self._ctx)
+        else:
+            desc = np.ascontiguousarray(desc, dtype=np.float32)
+            fn = self._lib.v3dm_set_left_f32 if left else self._lib.v3dm_set_right_f32
+            check(fn(self._ctx, int(n), ptr(desc, ctypes.c_float)), self._ctx)
+
+    def _guided(self, fn, M, thr, lpos, rpos, minScore, minRatio):
+        M = np.ascontiguousarray(np.asarray(M, np.float64).reshape(9))
+        lpos = np.ascontiguousarray(np.asarray(lpos, np.float32).reshape(-1, 2))
+        rpos = np.ascontiguousarray(np.asarray(rpos, np.float32).reshape(-1, 2))
+        pairs, count = self._buffer()
+        check(fn(self._ctx, ptr(M, ctypes.c_double), float(thr), ptr(lpos, ctypes.c_float),
+                 ptr(rpos, ctypes.c_float), minScore, minRatio, ptr(pairs, ctypes.c_int32), self._cap,
+                 ctypes.byref(count)), self._ctx)
+        return _as_list(pairs, count.value)
+
+
+def _as_list(pairs: np.ndarray, n: int) -> list[tuple[int, int]]:
+    return [(int(a), int(b)) for a, b in pairs[:n]]
