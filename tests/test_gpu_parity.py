@@ -77,8 +77,10 @@ def test_shapes_match_oracle(m, kernel, shape):
     nL, nR = shape
     say("shape", shape, "kernel", kernel)
     d = pair(0, 100 + nL % 97, nL, nR)
+    # the engine keeps the reference's rounding of counts down to multiples of 32
+    L32, R32 = d["L"][:nL - nL % 32], d["R"][:nR - nR % 32]
     for rule in [(0.8, 0.8), (0.9, 0.8), (0.8, -1.0)]:
-        assert run(m, d["L"], d["R"], *rule, kernel=kernel) == oracle_pairs(d["L"], d["R"], *rule)
+        assert run(m, d["L"], d["R"], *rule, kernel=kernel) == oracle_pairs(L32, R32, *rule)
 
 
 @pytest.mark.parametrize("kernel", [v.KERNEL_I8, v.KERNEL_PLAIN])
