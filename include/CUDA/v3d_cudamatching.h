@@ -1,0 +1,200 @@
+// -*- C++ -*-
+// Drop-in V3D_CUDA::SIFT_Matching for the MI355X engine.
+//
+// Same header path, class name, public members, argument meaning and
+// defaults as the reference (V3D/include/CUDA/v3d_cudamatching.h:15-67), so
+// SfM/match_sequence and SfM/match_using_voctree compile unchanged apart from
+// dropping their cublasInit()/cublasShutdown() calls.  Every member forwards
+// to the C ABI in v3dmatch.h.
+//
+// Differences a caller can observe:
+//  * device failures throw std::runtime_error (a std::exception; both callers
+//    catch it) instead of exiting the process (v3d_cudamatching.cuh:9-24);
+//  * runGuided*Matching without allocate(..., true) throws instead of using
+//    unallocated buffers;
+//  * the guided members are templates over the matrix / point types, so any
+//    type indexable as F[i][j] and p[0], p[1] works (V3D::Matrix3x3d and
+//    V3D::Vector2f included) without pulling in the V3D math headers;
+//  * the destructor releases the device state (the reference leaves it to
+//    deallocate()).
+#ifndef V3D_CUDA_MATCHING_H
+#define V3D_CUDA_MATCHING_H
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../v3dmatch.h"
+
+namespace V3D_CUDA
+{
+
+struct SIFT_Matching
+{
+   SIFT_Matching()
+      : _warnOnRounding(true), _useReductionPath(false), _useGuidedMatching(false), _nMaxLeftFeatures(0),
+        _nMaxRightFeatures(0), _nLeftFeatures(0), _nRightFeatures(0), _ctx(0)
+   {
+   }
+
+   ~SIFT_Matching() { v3dm_destroy(_ctx); }
+
+   SIFT_Matching(SIFT_Matching const &) = delete;
+   SIFT_Matching &operator=(SIFT_Matching const &) = delete;
+
+   void enableRoundingWarning(bool flag = true)
+   {
+      _warnOnRounding = flag;
+      if (_ctx) v3dm_set_rounding_warning(_ctx, flag);
+   }
+
+   void enableReductionPath(bool flag = true)
+   {
+      _useReductionPath = flag;
+      if (_ctx) v3dm_set_reduction_path(_ctx, flag);
+   }
+
+   void allocate(int nMaxLeftFeatures, int nMaxRightFeatures, bool useGuidedMatching = false)
+   {
+      ensureContext();
+      check(v3dm_allocate(_ctx, nMaxLeftFeatures, nMaxRightFeatures, useGuidedMatching ? 1 : 0));
+      _useGuidedMatching = useGuidedMatching;
+      _nMaxLeftFeatures = nMaxLeftFeatures - (nMaxLeftFeatures & 31);
+      _nMaxRightFeatures = nMaxRightFeatures - (nMaxRightFeatures & 31);
+      _nLeftFeatures = _nRightFeatures = 0;
+   }
+
+   void allocate(int nMaxFeatures, bool useGuidedMatching = false)
+   {
+      this->allocate(nMaxFeatures, nMaxFeatures, useGuidedMatching);
+   }
+
+   void deallocate()
+   {
+      if (_ctx) check(v3dm_deallocate(_ctx));
+   }
+
+   void setLeftFeatures(int nFeatures, float const *descriptors)
+   {
+      check(v3dm_set_left_f32(context(), nFeatures, descriptors));
+      refreshCounts();
+   }
+
+   void setRightFeatures(int nFeatures, float const *descriptors)
+   {
+      check(v3dm_set_right_f32(context(), nFeatures, descriptors));
+      refreshCounts();
+   }
+
+   // Additive: uint8 descriptors u (the reference's floats u/512).
+   void setLeftFeaturesU8(int nFeatures, std::uint8_t const *descriptors)
+   {
+      check(v3dm_set_left_u8(context(), nFeatures, descriptors));
+      refreshCounts();
+   }
+
+   void setRightFeaturesU8(int nFeatures, std::uint8_t const *descriptors)
+   {
+      check(v3dm_set_right_u8(context(), nFeatures, descriptors));
+      refreshCounts();
+   }
+
+   // Ratio test is disabled if minRatio < 0.
+   void findPutativeMatches(std::vector<std::pair<int, int> > &matches, float const minScore = 0.9f,
+                            float const minRatio = 0.8f)
+   {
+      std::vector<std::int32_t> buf(2 * (size_t)capacity());
+      int count = 0;
+      check(v3dm_find_putative_matches(context(), minScore, minRatio, buf.data(), capacity(), &count));
+      unpack(buf, count, matches);
+   }
+
+   template <class Matrix3x3, class Point2>
+   void runGuidedMatching(Matrix3x3 const &F, float const distanceThreshold, Point2 const *leftPositions,
+                          Point2 const *rightPositions, std::vector<std::pair<int, int> > &matches,
+                          float const minScore = 0.9f, float const minRatio = 0.8f)
+   {
+      guided(1, F, distanceThreshold, leftPositions, rightPositions, matches, minScore, minRatio);
+   }
+
+   template <class Matrix3x3, class Point2>
+   void runGuidedHomographyMatching(Matrix3x3 const &H, float const distanceThreshold, Point2 const *leftPositions,
+                                    Point2 const *rightPositions, std::vector<std::pair<int, int> > &matches,
+                                    float const minScore = 0.9f, float const minRatio = 0.8f)
+   {
+      guided(2, H, distanceThreshold, leftPositions, rightPositions, matches, minScore, minRatio);
+   }
+
+ protected:
+   bool _warnOnRounding;
+   bool _useReductionPath;
+   bool _useGuidedMatching;
+   int _nMaxLeftFeatures, _nMaxRightFeatures, _nLeftFeatures, _nRightFeatures;
+   v3dm_ctx *_ctx;
+
+ private:
+   void check(int rc) const
+   {
+      if (rc != V3DM_OK) throw std::runtime_error(std::string("SIFT_Matching: ") + v3dm_last_error(_ctx));
+   }
+
+   void ensureContext()
+   {
+      if (_ctx) return;
+      if (v3dm_create(0, &_ctx) != V3DM_OK)
+         throw std::runtime_error(std::string("SIFT_Matching: ") + v3dm_last_error(0));
+      v3dm_set_rounding_warning(_ctx, _warnOnRounding);
+      v3dm_set_reduction_path(_ctx, _useReductionPath);
+   }
+
+   v3dm_ctx *context()
+   {
+      if (!_ctx) throw std::runtime_error("SIFT_Matching: allocate() has not been called");
+      return _ctx;
+   }
+
+   int capacity() const { return _nMaxLeftFeatures > 0 ? _nMaxLeftFeatures : 1; }
+
+   void refreshCounts() { v3dm_get_counts(_ctx, &_nLeftFeatures, &_nRightFeatures); }
+
+   static void unpack(std::vector<std::int32_t> const &buf, int count, std::vector<std::pair<int, int> > &matches)
+   {
+      matches.clear();
+      matches.reserve(count);
+      for (int k = 0; k < count; ++k) matches.push_back(std::make_pair((int)buf[2 * k], (int)buf[2 * k + 1]));
+   }
+
+   template <class Matrix3x3, class Point2>
+   void guided(int kind, Matrix3x3 const &M, float thr, Point2 const *lp, Point2 const *rp,
+               std::vector<std::pair<int, int> > &matches, float minScore, float minRatio)
+   {
+      double m[9];
+      for (int i = 0; i < 3; ++i)
+         for (int j = 0; j < 3; ++j) m[3 * i + j] = M[i][j];
+      std::vector<float> l(2 * (size_t)_nLeftFeatures), r(2 * (size_t)_nRightFeatures);
+      for (int k = 0; k < _nLeftFeatures; ++k)
+      {
+         l[2 * k] = lp[k][0];
+         l[2 * k + 1] = lp[k][1];
+      }
+      for (int k = 0; k < _nRightFeatures; ++k)
+      {
+         r[2 * k] = rp[k][0];
+         r[2 * k + 1] = rp[k][1];
+      }
+      std::vector<std::int32_t> buf(2 * (size_t)capacity());
+      int count = 0;
+      int rc = kind == 1 ? v3dm_run_guided_matching(context(), m, thr, l.data(), r.data(), minScore, minRatio,
+                                                    buf.data(), capacity(), &count)
+                         : v3dm_run_guided_homography_matching(context(), m, thr, l.data(), r.data(), minScore,
+                                                               minRatio, buf.data(), capacity(), &count);
+      check(rc);
+      unpack(buf, count, matches);
+   }
+};
+
+} // end namespace V3D_CUDA
+
+#endif

@@ -233,3 +233,57 @@ def test_c2_8192_i8_equals_plain_and_oracle(m):
     src = d["src"]
     correct = sum(1 for k, l in i8 if src[l] == k)
     assert correct >= 0.95 * len(i8) and len(i8) > 0.3 * 8192
+
+
+def _lcg_positions(seed, nL, nR, src):
+    """Positions exactly as tests/cpp/wrapper_check.cpp makes them."""
+    M = (1 << 64) - 1
+    s = (seed * 2654435761 + 1) & M
+    f32 = np.float32
+
+    def nxt():
+        nonlocal s
+        s = (s * 6364136223846793005 + 1442695040888963407) & M
+        return f32(f32((s >> 33) % 100000) / f32(100.0))
+
+    lp = np.zeros((nL, 2), np.float32)
+    rp = np.zeros((nR, 2), np.float32)
+    for k in range(nL):
+        lp[k, 0] = nxt()
+        lp[k, 1] = nxt()
+    for k in range(nR):
+        rp[k, 0] = nxt()
+        rp[k, 1] = nxt()
+        if src[k] >= 0:
+            rp[k, 0] = lp[src[k], 0] + f32(30.0)
+            rp[k, 1] = lp[src[k], 1] + f32(k % 5 - 2)
+    return lp, rp
+
+
+def test_cpp_dropin_header_matches_oracle(gpu_available):
+    """The C++ SIFT_Matching drop-in (float descriptors, guided F then H)."""
+    import os
+    import subprocess
+
+    say("c++ drop-in header")
+    exe = os.path.join(os.path.dirname(__file__), "cpp", "wrapper_check")
+    nL, nR, seed = 992, 1216, 4242
+    out = subprocess.run([exe, str(nL), str(nR), str(seed)], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stderr
+    lists, cur = {}, None
+    for line in out.stdout.splitlines():
+        a, b = line.split()
+        if a in ("find", "guided", "homography"):
+            cur = a
+            lists[cur] = []
+        else:
+            lists[cur].append((int(a), int(b)))
+    d = v.synth_pair(seed, nL, nR, 0.4, 0.05)
+    lp, rp = _lcg_positions(seed, nL, nR, d["src"])
+    F = np.array([[0, 0, 0], [0, 0, -1], [0, 1, 0]], np.float64)
+    H = np.array([[1, 0, 30], [0, 1, 0], [0, 0, 1]], np.float64)
+    assert lists["find"] == oracle_pairs(d["L"], d["R"], 0.8, 0.8)
+    assert lists["guided"] == oracle_pairs(d["L"], d["R"], 0.8, 0.8, masks=[(1, F, 1.5, lp, rp)])
+    assert lists["homography"] == oracle_pairs(d["L"], d["R"], 0.8, 0.8,
+                                               masks=[(1, F, 1.5, lp, rp), (2, H, 2.0, lp, rp)])
+    assert len(lists["guided"]) < len(lists["find"])

@@ -44,10 +44,10 @@ namespace v3dm {
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 
-constexpr int kScanWaves = 4;       // waves per workgroup
+constexpr int kScanWaves = 8;       // waves per workgroup
 constexpr int kWaveRows = 64;       // left rows per wave (two 32-row MFMA blocks)
-constexpr int kTileM = kScanWaves * kWaveRows; // 256 left rows per tile
-constexpr int kTileN = 256;         // right rows per tile
+constexpr int kTileM = kScanWaves * kWaveRows; // 512 left rows per tile
+constexpr int kTileN = 512;         // right rows per tile
 constexpr int kBias = 1 << 21;      // 128^3
 
 __device__ inline int find_pair_by_tile(const PairDesc *pairs, int nPairs, int tile)
@@ -108,8 +108,8 @@ __device__ inline void top2_insert(uint64_t *t1, uint64_t *t2, uint64_t key)
    atomicMax((unsigned long long *)t2, (unsigned long long)(old < key ? old : key));
 }
 
-__device__ __noinline__ void survivor(const ScanArgs &a, const PairDesc &pd, uint32_t epoch, int il, int jr,
-                                      int32_t score)
+__device__ __forceinline__ void survivor(const ScanArgs &a, const PairDesc &pd, uint32_t epoch, int il, int jr,
+                                         int32_t score)
 {
    if (il >= pd.nL || jr >= pd.nR) return;
    for (int m = 0; m < a.nMasks; ++m)
@@ -121,7 +121,10 @@ __device__ __noinline__ void survivor(const ScanArgs &a, const PairDesc &pd, uin
    top2_insert(a.top1R + pd.slotR + jr, a.top2R + pd.slotR + jr, pack_key(epoch, (uint32_t)score, (uint32_t)il));
 }
 
-__device__ inline v4i ld16(const int8_t *p) { return *reinterpret_cast<const v4i *>(p); }
+__device__ inline v4i ld16(const int8_t *p)
+{
+   return *(__attribute__((address_space(1))) const v4i *)(p);
+}
 
 __device__ inline int max16(const v16i &x)
 {
@@ -133,85 +136,193 @@ __device__ inline int max16(const v16i &x)
    return max(max(max(m0, m1), max(m2, m3)), max(m4, x[15]));
 }
 
+// acc0/acc1 register v (0..31, block v / 16).  A variable-index vector
+// extract stays in registers (indexed register moves), no scratch.
+__device__ inline int select32(const v16i &a0, const v16i &a1, int v)
+{
+   const int x0 = a0[v & 15], x1 = a1[v & 15];
+   return v < 16 ? x0 : x1;
+}
+
+__device__ inline int sgpr(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
 // Operand packing.  Descriptor byte e of a row goes to lane half h = e / 64,
 // MFMA step s = (e % 64) / 16, byte e % 16 of the lane's 16-byte operand.
 // A (left) lane l carries row l % 32, B (right) lane l carries row l % 32, so
 // every MFMA step pairs the same 32 descriptor bytes of both rows and four
 // steps cover all 128.  Accumulator register v of lane l is
 // D[8*(v/4) + 4*(l/32) + v%4][l%32].
-__global__ __launch_bounds__(256) void k_i8_scan(ScanArgs a)
+//
+// Tiling.  A workgroup of 8 waves owns 512 left rows (64 per wave, held in
+// registers for the whole tile) and sweeps 512 right rows in steps of 32.
+// Each step's 32 right rows (4 KiB) are loaded once per workgroup into an
+// LDS stage and read by all 8 waves.  Global loads run kAhead steps ahead
+// in a register ring; a 3-deep LDS ring lets one barrier per step separate
+// the writes of step i+1 from the reads of step i-2 on the same stage.
+// LDS stage layout: 16-byte unit p = row * 8 + (chunk ^ (row & 7)), so the
+// fragment reads (32 rows, same chunk) hit distinct bank groups.
+
+constexpr int kStepCols = 32;                  // right rows per step
+constexpr int kStageBytes = kStepCols * kDim;  // 4 KiB
+constexpr int kLdsStages = 3;
+constexpr int kAhead = 4;                      // register-ring depth (steps)
+
+struct StageRing
 {
+   uint64_t v[kAhead]; // this lane's 8 bytes of steps i+1 .. i+kAhead
+   int s[kAhead];    // column sums (wave 0, lanes < 32)
+};
+
+typedef __attribute__((address_space(1))) const uint64_t gu64;
+typedef __attribute__((address_space(1))) const int32_t gi32;
+typedef __attribute__((address_space(1))) const v4i gv4i;
+
+// Loads this lane's 8 bytes of step `step` and one column sum (all lanes load
+// a sum so the load stream is uniform; wave 0 stores them).
+__device__ inline void stage_fetch(uint64_t &v, int &sum, const int8_t *Rt, const int32_t *St, int step, int wave,
+                                   int lane)
+{
+   const int q = wave * 64 + lane;         // 8-byte unit of the stage, 0..511
+   const int p = q >> 1, r = p >> 3, x = p & 7;
+   v = *(gu64 *)(Rt + (size_t)step * kStageBytes + r * kDim + ((x ^ (r & 7)) << 4) + ((q & 1) << 3));
+   sum = *(gi32 *)(St + step * kStepCols + (lane & (kStepCols - 1)));
+}
+
+__device__ inline void stage_store(uint64_t v, int sum, int8_t *sB, int32_t *sC, int wave, int lane)
+{
+   const int q = wave * 64 + lane;
+   *reinterpret_cast<uint64_t *>(sB + q * 8) = v;
+   if (wave == 0 && lane < kStepCols) sC[lane] = sum;
+}
+
+__device__ inline void lds_barrier()
+{
+   // LDS writes of this wave done, then the workgroup barrier.  Written out
+   // so no vmcnt wait is attached: the global-load ring stays in flight.
+   asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+__global__ __launch_bounds__(512) void k_i8_scan(ScanArgs a)
+{
+   __shared__ __attribute__((aligned(16))) int8_t sB[kLdsStages][kStageBytes];
+   __shared__ int32_t sC[kLdsStages][kStepCols];
+
    const int lane = threadIdx.x & 63;
    const int wave = threadIdx.x >> 6;
    const int tile = blockIdx.x;
-   const int p = find_pair_by_tile(a.pairs, a.nPairs, tile);
-   const PairDesc pd = a.pairs[p];
+   const int p = sgpr(find_pair_by_tile(a.pairs, a.nPairs, tile));
+   PairDesc pd = a.pairs[p];
+   // tile parameters are workgroup-uniform: keep them in scalar registers
+   pd.offL = ((int64_t)sgpr((int)(pd.offL >> 32)) << 32) | (uint32_t)sgpr((int)pd.offL);
+   pd.offR = ((int64_t)sgpr((int)(pd.offR >> 32)) << 32) | (uint32_t)sgpr((int)pd.offR);
+   pd.slotL = ((int64_t)sgpr((int)(pd.slotL >> 32)) << 32) | (uint32_t)sgpr((int)pd.slotL);
+   pd.slotR = ((int64_t)sgpr((int)(pd.slotR >> 32)) << 32) | (uint32_t)sgpr((int)pd.slotR);
+   pd.nL = sgpr(pd.nL);
+   pd.nR = sgpr(pd.nR);
+   pd.tilesN = sgpr(pd.tilesN);
+   pd.tileBase = sgpr(pd.tileBase);
    const int t = tile - pd.tileBase;
    const int tm = t / pd.tilesN, tn = t - tm * pd.tilesN;
    const int nLpad = (pd.nL + kRowPad - 1) / kRowPad * kRowPad;
    const int nRpad = (pd.nR + kRowPad - 1) / kRowPad * kRowPad;
-   const int l0 = tm * kTileM + wave * kWaveRows;
-   if (l0 >= nLpad) return; // wave-uniform; the kernel has no barriers
-   const uint32_t epoch = *a.epoch;
+   const int l0 = sgpr(tm * kTileM + wave * kWaveRows);
+   const bool active = l0 < nLpad; // wave-uniform; inactive waves still load and sync
+   const uint32_t epoch = (uint32_t)sgpr((int)*a.epoch);
    const int32_t thr = a.thr;
-
    const int half = lane >> 5, lrow = lane & 31;
+
+   const int c0 = sgpr(tn * kTileN);
+   const int nsteps = sgpr((min(c0 + kTileN, nRpad) - c0) / kStepCols); // multiple of kAhead
+   const int8_t *Rt = a.R + (size_t)(pd.offR + c0) * kDim;
+   const int32_t *St = a.Rsum + pd.offR + c0;
+
+   // Every tile has a multiple of kAhead steps (images are padded to 128
+   // rows), so the ring is filled and drained without branches; fetches past
+   // the last step re-read the last step and are never stored to a read stage.
+   StageRing ring;
+#pragma unroll
+   for (int j = 0; j < kAhead; ++j) stage_fetch(ring.v[j], ring.s[j], Rt, St, j, wave, lane);
+
    v4i A[2][4];
    v16i C[2];
-#pragma unroll
-   for (int b = 0; b < 2; ++b)
+   if (active)
    {
-      const int8_t *ap = a.L + (size_t)(pd.offL + l0 + 32 * b + lrow) * kDim + 64 * half;
 #pragma unroll
-      for (int s = 0; s < 4; ++s) A[b][s] = ld16(ap + 16 * s);
-#pragma unroll
-      for (int v = 0; v < 16; ++v)
-         C[b][v] = a.Lsum[pd.offL + l0 + 32 * b + 8 * (v >> 2) + 4 * half + (v & 3)] - kBias;
-   }
-
-   const int c0 = tn * kTileN;
-   const int c1 = min(c0 + kTileN, nRpad);
-   const int8_t *rbase = a.R + (size_t)pd.offR * kDim + 64 * half;
-   v4i B[4], Bn[4];
-   int col = c0 + lrow;
-#pragma unroll
-   for (int s = 0; s < 4; ++s) B[s] = ld16(rbase + (size_t)col * kDim + 16 * s);
-   int cs = a.Rsum[pd.offR + col];
-
-   for (int c = c0; c < c1; c += 32)
-   {
-      const int cn = c + 32 < c1 ? c + 32 : c; // prefetch the next step (re-read on the last)
-      const int coln = cn + lrow;
-#pragma unroll
-      for (int s = 0; s < 4; ++s) Bn[s] = ld16(rbase + (size_t)coln * kDim + 16 * s);
-      const int csn = a.Rsum[pd.offR + coln];
-
-      v16i acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[0][0], B[0], C[0], 0, 0, 0);
-      v16i acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[1][0], B[0], C[1], 0, 0, 0);
-#pragma unroll
-      for (int s = 1; s < 4; ++s)
+      for (int b = 0; b < 2; ++b)
       {
-         acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[0][s], B[s], acc0, 0, 0, 0);
-         acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[1][s], B[s], acc1, 0, 0, 0);
-      }
-      const int m = max(max16(acc0), max16(acc1));
-      const int lim = thr - cs; // score > thr  <=>  acc > thr - 128*sum(u_right)
-      if (__builtin_expect(__any(m > lim), 0))
-      {
+         const int8_t *ap = a.L + (size_t)(pd.offL + l0 + 32 * b + lrow) * kDim + 64 * half;
 #pragma unroll
-         for (int b = 0; b < 2; ++b)
+         for (int s = 0; s < 4; ++s) A[b][s] = ld16(ap + 16 * s);
+#pragma unroll
+         for (int q = 0; q < 4; ++q)
          {
-            const v16i &acc = b == 0 ? acc0 : acc1;
+            const v4i rs = *reinterpret_cast<const v4i *>(a.Lsum + pd.offL + l0 + 32 * b + 8 * q + 4 * half);
 #pragma unroll
-            for (int v = 0; v < 16; ++v)
-               if (acc[v] > lim)
-                  survivor(a, pd, epoch, l0 + 32 * b + 8 * (v >> 2) + 4 * half + (v & 3), col, acc[v] + cs);
+            for (int e = 0; e < 4; ++e) C[b][4 * q + e] = rs[e] - kBias;
          }
       }
+   }
+
+   // step 0 into stage 0
+   stage_store(ring.v[0], ring.s[0], sB[0], sC[0], wave, lane);
+   stage_fetch(ring.v[0], ring.s[0], Rt, St, min(kAhead, nsteps - 1), wave, lane);
+
+   for (int i0 = 0; i0 < nsteps; i0 += kAhead)
+   {
 #pragma unroll
-      for (int s = 0; s < 4; ++s) B[s] = Bn[s];
-      cs = csn;
-      col = coln;
+      for (int j = 0; j < kAhead; ++j)
+      {
+         const int i = i0 + j;
+         {
+            // step i+1 (ring slot (j+1) % kAhead) -> its LDS stage, then refill the slot.
+            // Stage (i+1) % 3 was last read at step i-2, before the previous barrier.
+            const int slot = (j + 1) % kAhead;
+            stage_store(ring.v[slot], ring.s[slot], sB[(i + 1) % kLdsStages], sC[(i + 1) % kLdsStages], wave, lane);
+            stage_fetch(ring.v[slot], ring.s[slot], Rt, St, min(i + 1 + kAhead, nsteps - 1), wave, lane);
+            lds_barrier();
+            if (active)
+            {
+               const int8_t *sb = sB[i % kLdsStages];
+               v4i B[4];
+#pragma unroll
+               for (int s = 0; s < 4; ++s)
+               {
+                  const int ch = 4 * half + s;
+                  B[s] = *reinterpret_cast<const v4i *>(sb + ((lrow * 8 + (ch ^ (lrow & 7))) << 4));
+               }
+               const int cs = sC[i % kLdsStages][lrow];
+               v16i acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[0][0], B[0], C[0], 0, 0, 0);
+               v16i acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[1][0], B[0], C[1], 0, 0, 0);
+#pragma unroll
+               for (int s = 1; s < 4; ++s)
+               {
+                  acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[0][s], B[s], acc0, 0, 0, 0);
+                  acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[1][s], B[s], acc1, 0, 0, 0);
+               }
+               const int m = max(max16(acc0), max16(acc1));
+               const int lim = thr - cs; // score > thr  <=>  acc > thr - 128*sum(u_right)
+               if (__builtin_expect(__any(m > lim), 0))
+               {
+                  // rare: exact top-2 updates for every surviving score of this lane
+                  uint32_t hits = 0;
+#pragma unroll
+                  for (int v = 0; v < 16; ++v)
+                  {
+                     hits |= acc0[v] > lim ? 1u << v : 0u;
+                     hits |= acc1[v] > lim ? 1u << (16 + v) : 0u;
+                  }
+                  const int col = c0 + i * kStepCols + lrow;
+                  while (hits)
+                  {
+                     const int v = __builtin_ctz(hits);
+                     hits &= hits - 1;
+                     const int il = l0 + 32 * (v >> 4) + 8 * ((v >> 2) & 3) + 4 * half + (v & 3);
+                     survivor(a, pd, epoch, il, col, select32(acc0, acc1, v) + cs);
+                  }
+               }
+            }
+         }
+      }
    }
 }
 
@@ -307,7 +418,7 @@ __global__ void k_mfma_probe(int32_t *out)
 hipError_t launch_i8_scan(const ScanArgs &a, hipStream_t st)
 {
    if (a.nTiles <= 0) return hipSuccess;
-   hipLaunchKernelGGL(k_i8_scan, dim3(a.nTiles), dim3(256), 0, st, a);
+   hipLaunchKernelGGL(k_i8_scan, dim3(a.nTiles), dim3(kScanWaves * 64), 0, st, a);
    return hipGetLastError();
 }
 

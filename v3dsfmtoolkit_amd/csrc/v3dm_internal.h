@@ -7,7 +7,7 @@
 namespace v3dm {
 
 constexpr int kDim = 128;          // descriptor length
-constexpr int kRowPad = 64;        // descriptor pools are padded to multiples of 64 rows
+constexpr int kRowPad = 128;       // descriptor pools are padded to multiples of 128 rows
 constexpr int kMaxMasks = 16;      // guided masks accumulated between two plain finds
 constexpr int kMaxLocalIndex = (1 << 21) - 1; // per-image feature limit of the packed keys
 constexpr uint32_t kEpochLimit = (1u << 20) - 2;
