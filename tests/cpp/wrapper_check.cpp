@@ -80,7 +80,7 @@ int main(int argc, char **argv)
       std::vector<std::pair<int, int> > matches;
       matcher.findPutativeMatches(matches, 0.8f, 0.8f);
       print("find", matches);
-      matcher.runGuidedMatching(F, 1.5f, &lp[0], &rp[0], matches, 0.8f, 0.8f);
+      matcher.runGuidedMatching(F, 1.2f, &lp[0], &rp[0], matches, 0.8f, 0.8f);
       print("guided", matches);
       matcher.runGuidedHomographyMatching(H, 2.0f, &lp[0], &rp[0], matches, 0.8f, 0.8f);
       print("homography", matches);

@@ -283,7 +283,7 @@ def test_cpp_dropin_header_matches_oracle(gpu_available):
     F = np.array([[0, 0, 0], [0, 0, -1], [0, 1, 0]], np.float64)
     H = np.array([[1, 0, 30], [0, 1, 0], [0, 0, 1]], np.float64)
     assert lists["find"] == oracle_pairs(d["L"], d["R"], 0.8, 0.8)
-    assert lists["guided"] == oracle_pairs(d["L"], d["R"], 0.8, 0.8, masks=[(1, F, 1.5, lp, rp)])
+    assert lists["guided"] == oracle_pairs(d["L"], d["R"], 0.8, 0.8, masks=[(1, F, 1.2, lp, rp)])
     assert lists["homography"] == oracle_pairs(d["L"], d["R"], 0.8, 0.8,
-                                               masks=[(1, F, 1.5, lp, rp), (2, H, 2.0, lp, rp)])
+                                               masks=[(1, F, 1.2, lp, rp), (2, H, 2.0, lp, rp)])
     assert len(lists["guided"]) < len(lists["find"])

@@ -37,6 +37,8 @@
 // key, whose score is the reference's nextVal (a duplicate best counts).
 // Keys of older jobs carry an older epoch and read as empty, so no clearing
 // pass is needed between jobs.
+#include <type_traits>
+
 #include "v3dm_internal.h"
 
 namespace v3dm {
@@ -44,9 +46,9 @@ namespace v3dm {
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 
-constexpr int kScanWaves = 8;       // waves per workgroup
+constexpr int kScanWaves = 4;       // waves per workgroup (two workgroups share a CU)
 constexpr int kWaveRows = 64;       // left rows per wave (two 32-row MFMA blocks)
-constexpr int kTileM = kScanWaves * kWaveRows; // 512 left rows per tile
+constexpr int kTileM = kScanWaves * kWaveRows; // 256 left rows per tile
 constexpr int kTileN = 512;         // right rows per tile
 constexpr int kBias = 1 << 21;      // 128^3
 
@@ -153,65 +155,48 @@ __device__ inline int sgpr(int x) { return __builtin_amdgcn_readfirstlane(x); }
 // steps cover all 128.  Accumulator register v of lane l is
 // D[8*(v/4) + 4*(l/32) + v%4][l%32].
 //
-// Tiling.  A workgroup of 8 waves owns 512 left rows (64 per wave, held in
-// registers for the whole tile) and sweeps 512 right rows in steps of 32.
-// Each step's 32 right rows (4 KiB) are loaded once per workgroup into an
-// LDS stage and read by all 8 waves.  Global loads run kAhead steps ahead
-// in a register ring; a 3-deep LDS ring lets one barrier per step separate
-// the writes of step i+1 from the reads of step i-2 on the same stage.
-// LDS stage layout: 16-byte unit p = row * 8 + (chunk ^ (row & 7)), so the
-// fragment reads (32 rows, same chunk) hit distinct bank groups.
+// Tiling.  A workgroup of 4 waves owns 256 left rows (64 per wave, held in
+// VGPRs for the whole tile) and sweeps 512 right rows in steps of 32; two
+// workgroups share a CU, so one computes while the other waits at a
+// barrier.  The right rows are staged through LDS once per workgroup and
+// read by all 4 waves.  Staging works in groups of 4 steps (16 KiB): while the waves
+// compute group g from one third of a 3-group LDS ring, the group g+1 data
+// (loaded during group g-1) is written to the next third and group g+2 is
+// fetched into registers.  One barrier per group; the third written at group
+// g was last read at group g-2, which every wave finished before the
+// barrier of group g-1.  Within a group the waves run free, and each wave
+// reads the fragments of step s+1 while its MFMAs of step s execute.
+// LDS unit layout: 16-byte unit p = row * 8 + (chunk ^ swz(row)),
+// swz(r) = (r ^ r >> 3) & 7, so the 32 rows of one fragment read (same
+// chunk) fall in distinct bank groups for 8- and 16-lane LDS service.
 
-constexpr int kStepCols = 32;                  // right rows per step
-constexpr int kStageBytes = kStepCols * kDim;  // 4 KiB
-constexpr int kLdsStages = 3;
-constexpr int kAhead = 4;                      // register-ring depth (steps)
+constexpr int kStepCols = 32;                      // right rows per step
+constexpr int kStageBytes = kStepCols * kDim;      // 4 KiB
+constexpr int kGroup = 4;                          // steps per barrier
+constexpr int kGroupBytes = kGroup * kStageBytes;  // 16 KiB
+constexpr int kThirds = 3;
+constexpr int kRsrcWord3 = 0x00020000;             // raw buffer, 32-bit data format
 
-struct StageRing
-{
-   uint64_t v[kAhead]; // this lane's 8 bytes of steps i+1 .. i+kAhead
-   int s[kAhead];    // column sums (wave 0, lanes < 32)
-};
-
-typedef __attribute__((address_space(1))) const uint64_t gu64;
-typedef __attribute__((address_space(1))) const int32_t gi32;
-typedef __attribute__((address_space(1))) const v4i gv4i;
-
-// Loads this lane's 8 bytes of step `step` and one column sum (all lanes load
-// a sum so the load stream is uniform; wave 0 stores them).
-__device__ inline void stage_fetch(uint64_t &v, int &sum, const int8_t *Rt, const int32_t *St, int step, int wave,
-                                   int lane)
-{
-   const int q = wave * 64 + lane;         // 8-byte unit of the stage, 0..511
-   const int p = q >> 1, r = p >> 3, x = p & 7;
-   v = *(gu64 *)(Rt + (size_t)step * kStageBytes + r * kDim + ((x ^ (r & 7)) << 4) + ((q & 1) << 3));
-   sum = *(gi32 *)(St + step * kStepCols + (lane & (kStepCols - 1)));
-}
-
-__device__ inline void stage_store(uint64_t v, int sum, int8_t *sB, int32_t *sC, int wave, int lane)
-{
-   const int q = wave * 64 + lane;
-   *reinterpret_cast<uint64_t *>(sB + q * 8) = v;
-   if (wave == 0 && lane < kStepCols) sC[lane] = sum;
-}
+__device__ inline int swz(int r) { return (r ^ (r >> 3)) & 7; }
 
 __device__ inline void lds_barrier()
 {
    // LDS writes of this wave done, then the workgroup barrier.  Written out
-   // so no vmcnt wait is attached: the global-load ring stays in flight.
+   // so no vmcnt wait is attached: the register ring stays in flight.
    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
-__global__ __launch_bounds__(512) void k_i8_scan(ScanArgs a)
+__global__ __launch_bounds__(kScanWaves * 64) __attribute__((amdgpu_waves_per_eu(2))) void k_i8_scan(ScanArgs a)
 {
-   __shared__ __attribute__((aligned(16))) int8_t sB[kLdsStages][kStageBytes];
-   __shared__ int32_t sC[kLdsStages][kStepCols];
+   __shared__ __attribute__((aligned(16))) int8_t sB[kThirds][kGroupBytes];
+   __shared__ __attribute__((aligned(16))) int32_t sC[kThirds][kGroup * kStepCols];
 
-   const int lane = threadIdx.x & 63;
-   const int wave = threadIdx.x >> 6;
+   const int tid = threadIdx.x;
+   const int lane = tid & 63;
+   const int wave = tid >> 6;
    const int tile = blockIdx.x;
-   const int p = sgpr(find_pair_by_tile(a.pairs, a.nPairs, tile));
-   PairDesc pd = a.pairs[p];
+   PairDesc pd = a.pair0; // single pair: straight from the kernel arguments
+   if (a.nPairs > 1) pd = a.pairs[sgpr(find_pair_by_tile(a.pairs, a.nPairs, tile))];
    // tile parameters are workgroup-uniform: keep them in scalar registers
    pd.offL = ((int64_t)sgpr((int)(pd.offL >> 32)) << 32) | (uint32_t)sgpr((int)pd.offL);
    pd.offR = ((int64_t)sgpr((int)(pd.offR >> 32)) << 32) | (uint32_t)sgpr((int)pd.offR);
@@ -227,21 +212,58 @@ __global__ __launch_bounds__(512) void k_i8_scan(ScanArgs a)
    const int nRpad = (pd.nR + kRowPad - 1) / kRowPad * kRowPad;
    const int l0 = sgpr(tm * kTileM + wave * kWaveRows);
    const bool active = l0 < nLpad; // wave-uniform; inactive waves still load and sync
-   const uint32_t epoch = (uint32_t)sgpr((int)*a.epoch);
    const int32_t thr = a.thr;
    const int half = lane >> 5, lrow = lane & 31;
 
    const int c0 = sgpr(tn * kTileN);
-   const int nsteps = sgpr((min(c0 + kTileN, nRpad) - c0) / kStepCols); // multiple of kAhead
+   const int nsteps = sgpr((min(c0 + kTileN, nRpad) - c0) / kStepCols); // multiple of kGroup
+   const int ngroups = nsteps / kGroup;
    const int8_t *Rt = a.R + (size_t)(pd.offR + c0) * kDim;
    const int32_t *St = a.Rsum + pd.offR + c0;
+   const __amdgpu_buffer_rsrc_t rsB = __builtin_amdgcn_make_buffer_rsrc((void *)Rt, (short)0, nsteps * kStageBytes,
+                                                                         kRsrcWord3);
+   const __amdgpu_buffer_rsrc_t rsS = __builtin_amdgcn_make_buffer_rsrc((void *)St, (short)0,
+                                                                         nsteps * kStepCols * 4, kRsrcWord3);
+   uint64_t *region = a.surv + ((size_t)tile * kScanWaves + wave) * a.survCap;
+   int nsurv = 0; // survivors this wave found in this tile (wave-uniform)
 
-   // Every tile has a multiple of kAhead steps (images are padded to 128
-   // rows), so the ring is filled and drained without branches; fetches past
-   // the last step re-read the last step and are never stored to a read stage.
-   StageRing ring;
+   // This thread's 16-byte units of a group (1024 units): global offset, LDS offset.
+   constexpr int kUnits = kGroupBytes / 16 / (kScanWaves * 64);
+   int voff[kUnits], loff[kUnits];
 #pragma unroll
-   for (int j = 0; j < kAhead; ++j) stage_fetch(ring.v[j], ring.s[j], Rt, St, j, wave, lane);
+   for (int j = 0; j < kUnits; ++j)
+   {
+      const int q = tid + kScanWaves * 64 * j, st = q >> 8, pu = q & 255, r = pu >> 3, x = pu & 7;
+      voff[j] = st * kStageBytes + r * kDim + ((x ^ swz(r)) << 4);
+      loff[j] = st * kStageBytes + pu * 16;
+   }
+   typedef unsigned int v2u __attribute__((ext_vector_type(2)));
+   // Two register rings: group g+1 is stored to LDS while group g+3 is
+   // fetched, so every fetch has two groups of compute to arrive, and the
+   // prologue issues groups 0 and 1 together with the A rows (one round trip).
+   v4i ring[2][kUnits];
+   v2u ringS[2] = {{0u, 0u}, {0u, 0u}};
+   auto fetch = [&](auto slotc, int g) {
+      constexpr int slot = decltype(slotc)::value;
+      const int gc = min(g, ngroups - 1);
+#pragma unroll
+      for (int j = 0; j < kUnits; ++j)
+         ring[slot][j] = __builtin_amdgcn_raw_buffer_load_b128(rsB, voff[j], gc * kGroupBytes, 0);
+      // every wave loads the sums: a uniform load count keeps the vmcnt waits exact
+      ringS[slot] = __builtin_amdgcn_raw_buffer_load_b64(rsS, lane * 8, gc * kGroup * kStepCols * 4, 0);
+   };
+   auto store = [&](auto slotc, int third) {
+      constexpr int slot = decltype(slotc)::value;
+#pragma unroll
+      for (int j = 0; j < kUnits; ++j) *reinterpret_cast<v4i *>(&sB[third][loff[j]]) = ring[slot][j];
+      // every wave writes the same sums (no branch, so the vmcnt waits stay exact)
+      *reinterpret_cast<v2u *>(&sC[third][2 * lane]) = ringS[slot];
+   };
+   const std::integral_constant<int, 0> R0;
+   const std::integral_constant<int, 1> R1;
+
+   fetch(R0, 0);
+   fetch(R1, 1);
 
    v4i A[2][4];
    v16i C[2];
@@ -256,73 +278,144 @@ __global__ __launch_bounds__(512) void k_i8_scan(ScanArgs a)
 #pragma unroll
          for (int q = 0; q < 4; ++q)
          {
-            const v4i rs = *reinterpret_cast<const v4i *>(a.Lsum + pd.offL + l0 + 32 * b + 8 * q + 4 * half);
+            const v4i rs = *(__attribute__((address_space(1))) const v4i *)(a.Lsum + pd.offL + l0 + 32 * b + 8 * q +
+                                                                            4 * half);
 #pragma unroll
             for (int e = 0; e < 4; ++e) C[b][4 * q + e] = rs[e] - kBias;
          }
       }
    }
-
-   // step 0 into stage 0
-   stage_store(ring.v[0], ring.s[0], sB[0], sC[0], wave, lane);
-   stage_fetch(ring.v[0], ring.s[0], Rt, St, min(kAhead, nsteps - 1), wave, lane);
-
-   for (int i0 = 0; i0 < nsteps; i0 += kAhead)
-   {
+   int fragoff[4]; // this lane's fragment units within a stage
 #pragma unroll
-      for (int j = 0; j < kAhead; ++j)
+   for (int k = 0; k < 4; ++k) fragoff[k] = (lrow * 8 + ((4 * half + k) ^ swz(lrow))) << 4;
+
+   store(R0, 0);
+   fetch(R0, 2);
+
+   v4i Bc[4]; // fragments of the next step to compute
+   auto group = [&](auto slotc, int g) {
+      // ring slot `slotc` holds group g+1: stage it, then refill with group g+3
+      store(slotc, (g + 1) % kThirds);
+      fetch(slotc, g + 3);
+      lds_barrier();
+      if (!active) return;
+      const int8_t *sb = sB[g % kThirds];
+      const int32_t *sc = sC[g % kThirds];
+      if (g == 0)
       {
-         const int i = i0 + j;
+#pragma unroll
+         for (int k = 0; k < 4; ++k) Bc[k] = *reinterpret_cast<const v4i *>(sb + fragoff[k]);
+      }
+#pragma unroll
+      for (int s = 0; s < kGroup; ++s)
+      {
+         // Prefetch the next step's fragments; after the barrier of group g the
+         // stage of group g+1 is complete too (written before that barrier).
+         v4i Bn[4];
+         if (s + 1 < kGroup)
          {
-            // step i+1 (ring slot (j+1) % kAhead) -> its LDS stage, then refill the slot.
-            // Stage (i+1) % 3 was last read at step i-2, before the previous barrier.
-            const int slot = (j + 1) % kAhead;
-            stage_store(ring.v[slot], ring.s[slot], sB[(i + 1) % kLdsStages], sC[(i + 1) % kLdsStages], wave, lane);
-            stage_fetch(ring.v[slot], ring.s[slot], Rt, St, min(i + 1 + kAhead, nsteps - 1), wave, lane);
-            lds_barrier();
-            if (active)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) Bn[k] = *reinterpret_cast<const v4i *>(sb + (s + 1) * kStageBytes + fragoff[k]);
+         }
+         else if (g + 1 < ngroups)
+         {
+            const int8_t *sn = sB[(g + 1) % kThirds];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) Bn[k] = *reinterpret_cast<const v4i *>(sn + fragoff[k]);
+         }
+         const int cs = sc[s * kStepCols + lrow];
+         v16i acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[0][0], Bc[0], C[0], 0, 0, 0);
+         v16i acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[1][0], Bc[0], C[1], 0, 0, 0);
+#pragma unroll
+         for (int k = 1; k < 4; ++k)
+         {
+            acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[0][k], Bc[k], acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[1][k], Bc[k], acc1, 0, 0, 0);
+         }
+         const int m = max(max16(acc0), max16(acc1));
+         const int lim = thr - cs; // score > thr  <=>  acc > thr - 128*sum(u_right)
+         if (__builtin_expect(__any(m > lim), 0))
+         {
+            // rare: append every surviving score of the wave to its region
+            // (plain stores, nothing waits on them); k_i8_insert applies them.
+            const int jrel = (g * kGroup + s) * kStepCols + lrow;
+            const bool colOk = c0 + jrel < pd.nR;
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
             {
-               const int8_t *sb = sB[i % kLdsStages];
-               v4i B[4];
+               const v16i &acc = b == 0 ? acc0 : acc1;
+               if (!__any(max16(acc) > lim)) continue;
 #pragma unroll
-               for (int s = 0; s < 4; ++s)
+               for (int v = 0; v < 16; ++v)
                {
-                  const int ch = 4 * half + s;
-                  B[s] = *reinterpret_cast<const v4i *>(sb + ((lrow * 8 + (ch ^ (lrow & 7))) << 4));
-               }
-               const int cs = sC[i % kLdsStages][lrow];
-               v16i acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[0][0], B[0], C[0], 0, 0, 0);
-               v16i acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[1][0], B[0], C[1], 0, 0, 0);
-#pragma unroll
-               for (int s = 1; s < 4; ++s)
-               {
-                  acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[0][s], B[s], acc0, 0, 0, 0);
-                  acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[1][s], B[s], acc1, 0, 0, 0);
-               }
-               const int m = max(max16(acc0), max16(acc1));
-               const int lim = thr - cs; // score > thr  <=>  acc > thr - 128*sum(u_right)
-               if (__builtin_expect(__any(m > lim), 0))
-               {
-                  // rare: exact top-2 updates for every surviving score of this lane
-                  uint32_t hits = 0;
-#pragma unroll
-                  for (int v = 0; v < 16; ++v)
-                  {
-                     hits |= acc0[v] > lim ? 1u << v : 0u;
-                     hits |= acc1[v] > lim ? 1u << (16 + v) : 0u;
-                  }
-                  const int col = c0 + i * kStepCols + lrow;
-                  while (hits)
-                  {
-                     const int v = __builtin_ctz(hits);
-                     hits &= hits - 1;
-                     const int il = l0 + 32 * (v >> 4) + 8 * ((v >> 2) & 3) + 4 * half + (v & 3);
-                     survivor(a, pd, epoch, il, col, select32(acc0, acc1, v) + cs);
-                  }
+                  const int irel = 32 * b + 8 * (v >> 2) + 4 * half + (v & 3);
+                  const bool h = acc[v] > lim && colOk && l0 + irel < pd.nL;
+                  const uint64_t ball = __ballot(h);
+                  if (ball == 0) continue;
+                  const int slot = nsurv + __builtin_amdgcn_mbcnt_hi((uint32_t)(ball >> 32),
+                                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)ball, 0));
+                  if (h && slot < a.survCap)
+                     region[slot] = ((uint64_t)(uint32_t)(acc[v] + cs) << 32) | ((uint32_t)irel << 16) |
+                                    (uint32_t)jrel;
+                  nsurv += __builtin_popcountll(ball); // counts past survCap: region overflowed
                }
             }
          }
+         if (s + 1 < kGroup || g + 1 < ngroups)
+         {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) Bc[k] = Bn[k];
+         }
       }
+   };
+   for (int g = 0; g < ngroups; g += 2)
+   {
+      group(R1, g);
+      if (g + 1 < ngroups) group(R0, g + 1);
+   }
+   if (lane == 0) a.survCount[(size_t)tile * kScanWaves + wave] = nsurv;
+}
+
+// Applies the survivors of one (tile, wave) region per wave: exact top-2
+// insertion into the left and right key arrays (guided masks checked here).
+// A region whose count exceeds its capacity (possible only when very many
+// scores of one 64 x 512 block pass the filter, e.g. near-identical images)
+// is recomputed here from the descriptors, so the result stays exact.
+__global__ __launch_bounds__(256) void k_i8_insert(ScanArgs a)
+{
+   const int lane = threadIdx.x & 63;
+   const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+   if (r >= a.nTiles * kScanWaves) return;
+   const int cnt = a.survCount[r];
+   if (cnt == 0) return;
+   const int tile = r / kScanWaves, wave = r - tile * kScanWaves;
+   const int p = find_pair_by_tile(a.pairs, a.nPairs, tile);
+   const PairDesc pd = a.pairs[p];
+   const int t = tile - pd.tileBase;
+   const int tm = t / pd.tilesN, tn = t - tm * pd.tilesN;
+   const int l0 = tm * kTileM + wave * kWaveRows, c0 = tn * kTileN;
+   const uint32_t epoch = *a.epoch;
+   if (cnt <= a.survCap)
+   {
+      const uint64_t *region = a.surv + (size_t)r * a.survCap;
+      for (int e = lane; e < cnt; e += 64)
+      {
+         const uint64_t w = region[e];
+         survivor(a, pd, epoch, l0 + (int)((w >> 16) & 0xFFFF), c0 + (int)(w & 0xFFFF), (int32_t)(w >> 32));
+      }
+      return;
+   }
+   // overflowed region: every (left, right) of the block, exact u8 dot products
+   const int nRows = min(kWaveRows, pd.nL - l0), nCols = min(kTileN, pd.nR - c0);
+   for (int e = lane; e < nRows * nCols; e += 64)
+   {
+      const int il = l0 + e / nCols, jr = c0 + e % nCols;
+      const uint32_t *x = reinterpret_cast<const uint32_t *>(a.L + (size_t)(pd.offL + il) * kDim);
+      const uint32_t *y = reinterpret_cast<const uint32_t *>(a.R + (size_t)(pd.offR + jr) * kDim);
+      uint32_t acc = 0;
+      for (int k = 0; k < kDim / 4; ++k) // stored bytes are u ^ 0x80
+         acc = __builtin_amdgcn_udot4(x[k] ^ 0x80808080u, y[k] ^ 0x80808080u, acc, false);
+      if ((int32_t)acc > a.thr) survivor(a, pd, epoch, il, jr, (int32_t)acc);
    }
 }
 
@@ -421,6 +514,16 @@ hipError_t launch_i8_scan(const ScanArgs &a, hipStream_t st)
    hipLaunchKernelGGL(k_i8_scan, dim3(a.nTiles), dim3(kScanWaves * 64), 0, st, a);
    return hipGetLastError();
 }
+
+hipError_t launch_i8_insert(const ScanArgs &a, hipStream_t st)
+{
+   if (a.nTiles <= 0) return hipSuccess;
+   const int regions = a.nTiles * kScanWaves;
+   hipLaunchKernelGGL(k_i8_insert, dim3((regions + 3) / 4), dim3(256), 0, st, a);
+   return hipGetLastError();
+}
+
+int scan_waves() { return kScanWaves; }
 
 hipError_t launch_i8_finalize(const FinalizeArgs &a, hipStream_t st)
 {

@@ -18,6 +18,8 @@
 namespace v3dm {
 hipError_t launch_i8_scan(const ScanArgs &a, hipStream_t st);
 hipError_t launch_i8_finalize(const FinalizeArgs &a, hipStream_t st);
+hipError_t launch_i8_insert(const ScanArgs &a, hipStream_t st);
+int scan_waves();
 hipError_t launch_clear_keys(uint64_t *keys, int64_t n, uint32_t *epoch, uint32_t *done, hipStream_t st);
 hipError_t launch_mfma_probe(int32_t *out, hipStream_t st);
 int scan_tile_m();
@@ -119,6 +121,11 @@ struct v3dm_ctx
    bool pendingAsync = false;
    int lastNL = 0;
 
+   // survivor regions of the fused scan
+   uint64_t *dSurv = nullptr;
+   int32_t *dSurvCount = nullptr;
+   int survCap = 64;
+
    // kernel timing: three events per job (before scan, after scan, after select)
    bool profile = false;
    int profJobs = 0;
@@ -197,6 +204,8 @@ void freeAll(v3dm_ctx *c)
    dfree(c->dPart);
    dfree(c->dColMax);
    dfree(c->dRowMax);
+   dfree(c->dSurv);
+   dfree(c->dSurvCount);
    if (c->hMatch) (void)hipHostFree(c->hMatch);
    if (c->hFlag) (void)hipHostFree(c->hFlag);
    c->hMatch = nullptr;
@@ -367,6 +376,7 @@ int enqueueI8(v3dm_ctx *c, float minScore, float minRatio)
    }
    const PairDesc &pd = c->hPairUploaded;
    ScanArgs a{};
+   a.pair0 = pd;
    a.L = c->L.dI8;
    a.Lsum = c->L.dSum;
    a.R = c->R.dI8;
@@ -382,9 +392,13 @@ int enqueueI8(v3dm_ctx *c, float minScore, float minRatio)
    a.epoch = c->dEpoch;
    a.masks = c->dMasks;
    a.nMasks = (int)c->masks.size();
+   a.surv = c->dSurv;
+   a.survCount = c->dSurvCount;
+   a.survCap = c->survCap;
    if ((rc = profMark(c, 0))) return rc;
    HIPCHK(c, launch_i8_scan(a, c->stream));
    if ((rc = profMark(c, 1))) return rc;
+   HIPCHK(c, launch_i8_insert(a, c->stream));
    FinalizeArgs f{};
    f.pairs = c->dPair;
    f.nPairs = 1;
@@ -649,6 +663,13 @@ int v3dm_allocate(v3dm_ctx *c, int nMaxLeft, int nMaxRight, int useGuided)
    HIPCHK(c, hipMalloc((void **)&c->dMasks, sizeof(Mask) * kMaxMasks));
    HIPCHK(c, hipMalloc((void **)&c->dColMax, sizeof(int32_t) * nPadL));
    HIPCHK(c, hipMalloc((void **)&c->dRowMax, sizeof(int32_t) * std::max(roundUp(nMaxRight, kRowPad), kRowPad)));
+   {
+      const int64_t tiles = (int64_t)((roundUp(std::max(nMaxLeft, 1), kRowPad) + scan_tile_m() - 1) / scan_tile_m()) *
+                            ((roundUp(std::max(nMaxRight, 1), kRowPad) + scan_tile_n() - 1) / scan_tile_n());
+      const int64_t regions = tiles * scan_waves();
+      HIPCHK(c, hipMalloc((void **)&c->dSurv, sizeof(uint64_t) * (size_t)regions * c->survCap));
+      HIPCHK(c, hipMalloc((void **)&c->dSurvCount, sizeof(int32_t) * (size_t)regions));
+   }
    HIPCHK(c, hipHostMalloc((void **)&c->hMatch, sizeof(int32_t) * nPadL, hipHostMallocDefault));
    HIPCHK(c, hipHostMalloc((void **)&c->hFlag, sizeof(int), hipHostMallocDefault));
    c->allocated = true;

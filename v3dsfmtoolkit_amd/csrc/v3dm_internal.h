@@ -48,6 +48,7 @@ __host__ __device__ inline uint32_t key_index(uint64_t k) { return (uint32_t)(kM
 
 struct ScanArgs
 {
+   PairDesc pair0;              // the pair itself when nPairs == 1 (no lookup)
    const int8_t *L;     // left pool, (u - 128) as int8, rows padded to kRowPad
    const int32_t *Lsum; // 128 * sum(u) per left row (0 for pad rows)
    const int8_t *R;
@@ -61,6 +62,9 @@ struct ScanArgs
    const uint32_t *epoch;       // device-resident epoch of the current job
    const Mask *masks;           // guided masks (single-pair jobs only)
    int nMasks;
+   uint64_t *surv;              // survivor regions, survCap entries per (tile, wave)
+   int32_t *survCount;          // entries written per region (may exceed survCap)
+   int survCap;
 };
 
 struct FinalizeArgs
