@@ -145,6 +145,53 @@ int v3dm_profile_enable(v3dm_ctx *ctx, int enable);
  * jobs recorded since profiling was enabled, and their number. */
 int v3dm_profile_read(v3dm_ctx *ctx, double *scanMs, double *selectMs, int *jobs);
 
+/* --- many image pairs in one job -------------------------------------------- */
+/* The reference matches one pair per findPutativeMatches call
+ * (SfM/match_sequence/src/match_sequence.cpp:157-203,
+ * SfM/match_using_voctree/src/match_using_voctree.cpp:277-323); these entry
+ * points run a whole list of pairs in one job with the same per-pair rule,
+ * so each pair's table equals findPutativeMatches on that pair. */
+
+/* Uploads nImages uint8 descriptor sets (rows concatenated in image order,
+ * counts[i] rows each).  Counts are rounded down to multiples of 32 like
+ * setLeftFeatures/setRightFeatures (v3d_cudamatching.cpp:68-77).  Clears any
+ * pair list. */
+int v3dm_batch_set_images_u8(v3dm_ctx *ctx, int nImages, const int32_t *counts, const uint8_t *desc);
+
+/* Sets the pairs to match: pairImages[2p] = left image, [2p+1] = right image. */
+int v3dm_batch_set_pairs(v3dm_ctx *ctx, int nPairs, const int32_t *pairImages);
+
+/* Runs the job.  matchOfLeft receives, for pair p, entries
+ * [offsets[p], offsets[p+1]) = per-left-feature right index or -1 (the
+ * reference's mutual list as a table); offsets has nPairs + 1 entries. */
+int v3dm_batch_match(v3dm_ctx *ctx, float minScore, float minRatio, int32_t *matchOfLeft, int64_t *offsets);
+
+/* Asynchronous form: enqueue, then fetch (waits). */
+int v3dm_batch_match_async(v3dm_ctx *ctx, float minScore, float minRatio);
+int v3dm_batch_fetch(v3dm_ctx *ctx, int32_t *matchOfLeft, int64_t *offsets);
+
+/* Scan tile variant: -1 = automatic (tuning cache named by the environment
+ * variable V3DM_TUNE_CACHE, else a timed trial of every variant on the
+ * first job); otherwise a variant index. */
+int v3dm_batch_set_variant(v3dm_ctx *ctx, int variant);
+int v3dm_batch_get_variant(const v3dm_ctx *ctx, int *variant);
+
+/* Times every variant on the current job (reps runs each), keeps the fastest;
+ * msPerVariant (may be NULL) receives v3dm_scan_variant_count() times. */
+int v3dm_batch_autotune(v3dm_ctx *ctx, float minScore, float minRatio, int reps, double *msPerVariant, int *best);
+
+/* Capture the job's kernels in a hipGraph and replay it while the pair set
+ * and thresholds stay the same. */
+int v3dm_batch_set_graph(v3dm_ctx *ctx, int enable);
+
+/* Pair count, total left features, scan tiles of the current variant, and
+ * descriptor pairs scored (sum of nL * nR) of the current job. */
+int v3dm_batch_info(const v3dm_ctx *ctx, int *nPairs, int64_t *nLeftTotal, int *nTiles, int64_t *pairsScored);
+
+/* Scan tile variants: count, and waves per workgroup / tile rows / tile columns. */
+int v3dm_scan_variant_count(void);
+int v3dm_scan_variant_info(int variant, int *waves, int *tileRows, int *tileCols);
+
 /* --- synthetic SIFT-like descriptors (host) -------------------------------- */
 
 /* Generates one image pair: nL left descriptors, nR right descriptors of

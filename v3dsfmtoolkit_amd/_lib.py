@@ -33,6 +33,7 @@ _i32_p = ctypes.POINTER(ctypes.c_int32)
 _u8_p = ctypes.POINTER(ctypes.c_uint8)
 _f32_p = ctypes.POINTER(ctypes.c_float)
 _f64_p = ctypes.POINTER(ctypes.c_double)
+_i64_p = ctypes.POINTER(ctypes.c_int64)
 _ctx_p = ctypes.c_void_p
 
 _SIGNATURES = {
@@ -62,6 +63,18 @@ _SIGNATURES = {
     "v3dm_synchronize": (ctypes.c_int, [_ctx_p]),
     "v3dm_profile_enable": (ctypes.c_int, [_ctx_p, ctypes.c_int]),
     "v3dm_profile_read": (ctypes.c_int, [_ctx_p, _f64_p, _f64_p, _c_int_p]),
+    "v3dm_batch_set_images_u8": (ctypes.c_int, [_ctx_p, ctypes.c_int, _i32_p, _u8_p]),
+    "v3dm_batch_set_pairs": (ctypes.c_int, [_ctx_p, ctypes.c_int, _i32_p]),
+    "v3dm_batch_match": (ctypes.c_int, [_ctx_p, ctypes.c_float, ctypes.c_float, _i32_p, _i64_p]),
+    "v3dm_batch_match_async": (ctypes.c_int, [_ctx_p, ctypes.c_float, ctypes.c_float]),
+    "v3dm_batch_fetch": (ctypes.c_int, [_ctx_p, _i32_p, _i64_p]),
+    "v3dm_batch_set_variant": (ctypes.c_int, [_ctx_p, ctypes.c_int]),
+    "v3dm_batch_get_variant": (ctypes.c_int, [_ctx_p, _c_int_p]),
+    "v3dm_batch_autotune": (ctypes.c_int, [_ctx_p, ctypes.c_float, ctypes.c_float, ctypes.c_int, _f64_p, _c_int_p]),
+    "v3dm_batch_set_graph": (ctypes.c_int, [_ctx_p, ctypes.c_int]),
+    "v3dm_batch_info": (ctypes.c_int, [_ctx_p, _c_int_p, _i64_p, _c_int_p, _i64_p]),
+    "v3dm_scan_variant_count": (ctypes.c_int, []),
+    "v3dm_scan_variant_info": (ctypes.c_int, [ctypes.c_int, _c_int_p, _c_int_p, _c_int_p]),
     "v3dm_synth_pair": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                        ctypes.c_int, _u8_p, _u8_p, _f32_p, _f32_p, _i32_p]),
     "v3dm_synth_seed": (ctypes.c_uint64, [ctypes.c_int, ctypes.c_int]),
@@ -148,4 +161,14 @@ def synth_pair(seed: int, n_left: int, n_right: int, rho: float = 0.4, sigma: fl
 def mfma_layout_probe(device: int = 0) -> np.ndarray:
     out = np.zeros((64, 16), np.int32)
     check(lib().v3dm_mfma_layout_probe(device, ptr(out, ctypes.c_int32)))
+    return out
+
+
+def scan_variants() -> list[tuple[int, int, int]]:
+    """(waves per workgroup, tile rows, tile columns) of each scan variant."""
+    out = []
+    for v in range(lib().v3dm_scan_variant_count()):
+        w, r, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(lib().v3dm_scan_variant_info(v, ctypes.byref(w), ctypes.byref(r), ctypes.byref(c)))
+        out.append((w.value, r.value, c.value))
     return out

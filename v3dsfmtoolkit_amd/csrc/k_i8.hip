@@ -46,10 +46,7 @@ namespace v3dm {
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 
-constexpr int kScanWaves = 4;       // waves per workgroup (two workgroups share a CU)
 constexpr int kWaveRows = 64;       // left rows per wave (two 32-row MFMA blocks)
-constexpr int kTileM = kScanWaves * kWaveRows; // 256 left rows per tile
-constexpr int kTileN = 512;         // right rows per tile
 constexpr int kBias = 1 << 21;      // 128^3
 
 __device__ inline int find_pair_by_tile(const PairDesc *pairs, int nPairs, int tile)
@@ -186,15 +183,28 @@ __device__ inline void lds_barrier()
    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
-__global__ __launch_bounds__(kScanWaves * 64) __attribute__((amdgpu_waves_per_eu(2))) void k_i8_scan(ScanArgs a)
+// One workgroup per tile, or (a.persistent) resident workgroups that take
+// tiles blockIdx.x, then gridDim.x + atomicAdd(tileCounter) until the tiles
+// run out; the next index is requested when a tile starts, so its latency
+// hides behind the tile.
+template <int WAVES, int TILE_N>
+__global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(2))) void k_i8_scan(ScanArgs a)
 {
+   constexpr int kScanWaves = WAVES;
+   constexpr int kTileM = WAVES * kWaveRows;
+   constexpr int kTileN = TILE_N;
    __shared__ __attribute__((aligned(16))) int8_t sB[kThirds][kGroupBytes];
    __shared__ __attribute__((aligned(16))) int32_t sC[kThirds][kGroup * kStepCols];
+   __shared__ int sNextTile;
 
    const int tid = threadIdx.x;
    const int lane = tid & 63;
    const int wave = tid >> 6;
-   const int tile = blockIdx.x;
+   int tile = blockIdx.x;
+   while (tile < a.nTiles)
+   {
+   uint32_t nextReq = 0;
+   if (a.persistent && tid == 0) nextReq = atomicAdd(a.tileCounter, 1u) + gridDim.x;
    PairDesc pd = a.pair0; // single pair: straight from the kernel arguments
    if (a.nPairs > 1) pd = a.pairs[sgpr(find_pair_by_tile(a.pairs, a.nPairs, tile))];
    // tile parameters are workgroup-uniform: keep them in scalar registers
@@ -374,6 +384,12 @@ __global__ __launch_bounds__(kScanWaves * 64) __attribute__((amdgpu_waves_per_eu
       if (g + 1 < ngroups) group(R0, g + 1);
    }
    if (lane == 0) a.survCount[(size_t)tile * kScanWaves + wave] = nsurv;
+   if (!a.persistent) break;
+   __syncthreads(); // every wave is done with this tile's LDS stages
+   if (tid == 0) sNextTile = (int)nextReq;
+   __syncthreads();
+   tile = sNextTile;
+   }
 }
 
 // Applies the survivors of one (tile, wave) region per wave: exact top-2
@@ -385,15 +401,15 @@ __global__ __launch_bounds__(256) void k_i8_insert(ScanArgs a)
 {
    const int lane = threadIdx.x & 63;
    const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
-   if (r >= a.nTiles * kScanWaves) return;
+   if (r >= a.nTiles * a.waves) return;
    const int cnt = a.survCount[r];
    if (cnt == 0) return;
-   const int tile = r / kScanWaves, wave = r - tile * kScanWaves;
+   const int tile = r / a.waves, wave = r - tile * a.waves;
    const int p = find_pair_by_tile(a.pairs, a.nPairs, tile);
    const PairDesc pd = a.pairs[p];
    const int t = tile - pd.tileBase;
    const int tm = t / pd.tilesN, tn = t - tm * pd.tilesN;
-   const int l0 = tm * kTileM + wave * kWaveRows, c0 = tn * kTileN;
+   const int l0 = tm * a.tileM + wave * kWaveRows, c0 = tn * a.tileN;
    const uint32_t epoch = *a.epoch;
    if (cnt <= a.survCap)
    {
@@ -406,7 +422,7 @@ __global__ __launch_bounds__(256) void k_i8_insert(ScanArgs a)
       return;
    }
    // overflowed region: every (left, right) of the block, exact u8 dot products
-   const int nRows = min(kWaveRows, pd.nL - l0), nCols = min(kTileN, pd.nR - c0);
+   const int nRows = min(kWaveRows, pd.nL - l0), nCols = min(a.tileN, pd.nR - c0);
    for (int e = lane; e < nRows * nCols; e += 64)
    {
       const int il = l0 + e / nCols, jr = c0 + e % nCols;
@@ -473,6 +489,7 @@ __global__ __launch_bounds__(256) void k_i8_finalize(FinalizeArgs a)
       if (prev == (uint32_t)gridDim.x - 1)
       {
          *a.done = 0;
+         *a.tileCounter = 0;
          *a.epoch = epoch + 1;
          __threadfence();
       }
@@ -488,6 +505,7 @@ __global__ void k_clear_keys(uint64_t *keys, int64_t n, uint32_t *epoch, uint32_
    {
       *epoch = 1;
       *done = 0;
+      done[1] = 0; // persistent scan's tile counter follows the done counter
    }
 }
 
@@ -508,22 +526,57 @@ __global__ void k_mfma_probe(int32_t *out)
 
 // ---- launchers ------------------------------------------------------------
 
-hipError_t launch_i8_scan(const ScanArgs &a, hipStream_t st)
+// Scan variants: (waves per workgroup, right rows per tile).
+struct ScanVariant
+{
+   int waves, tileN;
+   const void *fn;
+};
+static const ScanVariant kVariants[] = {
+   {4, 512, reinterpret_cast<const void *>(&k_i8_scan<4, 512>)},
+   {4, 1024, reinterpret_cast<const void *>(&k_i8_scan<4, 1024>)},
+   {8, 512, reinterpret_cast<const void *>(&k_i8_scan<8, 512>)},
+   {4, 256, reinterpret_cast<const void *>(&k_i8_scan<4, 256>)},
+};
+
+int scan_variant_count() { return (int)(sizeof(kVariants) / sizeof(kVariants[0])); }
+int scan_variant_waves(int v) { return kVariants[v].waves; }
+int scan_variant_tile_m(int v) { return kVariants[v].waves * kWaveRows; }
+int scan_variant_tile_n(int v) { return kVariants[v].tileN; }
+
+// Workgroups of variant v resident on the whole device (for persistent launches).
+int scan_variant_resident(int v, int device)
+{
+   int perCU = 0, cus = 0;
+   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kVariants[v].fn, kVariants[v].waves * 64, 0) !=
+       hipSuccess)
+      perCU = 1;
+   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) cus = 1;
+   return (perCU > 0 ? perCU : 1) * (cus > 0 ? cus : 1);
+}
+
+hipError_t launch_i8_scan(int variant, int grid, const ScanArgs &a, hipStream_t st)
 {
    if (a.nTiles <= 0) return hipSuccess;
-   hipLaunchKernelGGL(k_i8_scan, dim3(a.nTiles), dim3(kScanWaves * 64), 0, st, a);
+   switch (variant)
+   {
+   case 0: hipLaunchKernelGGL((k_i8_scan<4, 512>), dim3(grid), dim3(256), 0, st, a); break;
+   case 1: hipLaunchKernelGGL((k_i8_scan<4, 1024>), dim3(grid), dim3(256), 0, st, a); break;
+   case 2: hipLaunchKernelGGL((k_i8_scan<8, 512>), dim3(grid), dim3(512), 0, st, a); break;
+   case 3: hipLaunchKernelGGL((k_i8_scan<4, 256>), dim3(grid), dim3(256), 0, st, a); break;
+   default: return hipErrorInvalidValue;
+   }
    return hipGetLastError();
 }
 
 hipError_t launch_i8_insert(const ScanArgs &a, hipStream_t st)
 {
    if (a.nTiles <= 0) return hipSuccess;
-   const int regions = a.nTiles * kScanWaves;
+   const int regions = a.nTiles * a.waves;
    hipLaunchKernelGGL(k_i8_insert, dim3((regions + 3) / 4), dim3(256), 0, st, a);
    return hipGetLastError();
 }
 
-int scan_waves() { return kScanWaves; }
 
 hipError_t launch_i8_finalize(const FinalizeArgs &a, hipStream_t st)
 {
@@ -546,7 +599,6 @@ hipError_t launch_mfma_probe(int32_t *out, hipStream_t st)
    return hipGetLastError();
 }
 
-int scan_tile_m() { return kTileM; }
-int scan_tile_n() { return kTileN; }
+
 
 } // namespace v3dm

@@ -4,162 +4,13 @@
 // buffer ownership, the multiple-of-32 rounding and clamping of feature
 // counts, the stateful guided-matching contract, kernel dispatch and the
 // mutual-match list in ascending left order.
-#include <algorithm>
-#include <cmath>
-#include <cstdarg>
-#include <cstdio>
-#include <cstring>
-#include <string>
-#include <vector>
+#include "v3dm_ctx.h"
 
-#include "../../include/v3dmatch.h"
-#include "v3dm_internal.h"
-
-namespace v3dm {
-hipError_t launch_i8_scan(const ScanArgs &a, hipStream_t st);
-hipError_t launch_i8_finalize(const FinalizeArgs &a, hipStream_t st);
-hipError_t launch_i8_insert(const ScanArgs &a, hipStream_t st);
-int scan_waves();
-hipError_t launch_clear_keys(uint64_t *keys, int64_t n, uint32_t *epoch, uint32_t *done, hipStream_t st);
-hipError_t launch_mfma_probe(int32_t *out, hipStream_t st);
-int scan_tile_m();
-int scan_tile_n();
-hipError_t launch_sgemm_seq(int nL, int nR, const float *Lf, const float *Rf, float *S, hipStream_t st);
-hipError_t launch_mask_plain(int nL, int nR, float *S, const Mask *masks, int nMasks, hipStream_t st);
-hipError_t launch_fill_i32(int n, int32_t value, int32_t *out, hipStream_t st);
-size_t plain_part_bytes(int nL, int nR, int *nChunksOut, int *rowsPerOut);
-hipError_t launch_plain_select(int nL, int nR, const float *S, void *part, float minScore, float minRatio,
-                               int32_t *colMax, int32_t *rowMax, int32_t *out, hipStream_t st);
-hipError_t launch_prep_u8(const uint8_t *src, int n, int nPad, int8_t *dst, int32_t *sum, hipStream_t st);
-hipError_t launch_prep_f32(const float *src, int n, int nPad, int8_t *dst, int32_t *sum, int *flag,
-                           hipStream_t st);
-hipError_t launch_u8_to_f32(const uint8_t *src, int64_t count, float *dst, hipStream_t st);
-} // namespace v3dm
-
-using namespace v3dm;
-
-namespace {
-
+namespace v3dm_host {
 thread_local std::string g_createError;
-
-int roundUp(int n, int m) { return (n + m - 1) / m * m; }
-
-// Survivor threshold in exact integer score units (score = int * 2^-18); see
-// the proof at the top of k_i8.hip.
-int32_t survivorThreshold(float minScore, float minRatio)
-{
-   double t;
-   if (minRatio < 0)
-      t = (double)minScore;
-   else if (minRatio > 0)
-   {
-      double tr = 1.0 - (1.0 - (double)minScore) / ((double)minRatio * (1.0 - 1e-5));
-      t = std::min((double)minScore, tr);
-   }
-   else
-      return -1; // minRatio == 0 or NaN: keep every score
-   if (!(t == t)) return -1;
-   double x = std::floor(t * 262144.0);
-   if (x < -1.0) return -1;
-   if (x > 2147483000.0) return 2147483000;
-   return (int32_t)x;
 }
 
-struct Side
-{
-   int n = 0;
-   int nMax = 0;
-   bool quantized = false; // int8 pool holds these features exactly
-   bool isU8 = false;      // last upload was uint8
-   bool f32Valid = false;  // dF32 holds the float form
-   uint8_t *dRawU8 = nullptr;
-   float *dF32 = nullptr; // float form (upload buffer for float input)
-   int8_t *dI8 = nullptr;
-   int32_t *dSum = nullptr;
-   uint64_t *dTop1 = nullptr, *dTop2 = nullptr;
-   float *dPos = nullptr; // kMaxMasks position sets (guided)
-};
-
-} // namespace
-
-struct v3dm_ctx
-{
-   int device = 0;
-   hipStream_t stream = nullptr;
-   std::string err;
-   bool warnRounding = true;
-   bool useReduction = false;
-   int kernelSel = V3DM_KERNEL_AUTO;
-   int lastKernel = 0;
-   bool allocated = false;
-   bool guided = false;
-   Side L, R;
-
-   PairDesc *dPair = nullptr;
-   PairDesc hPairUploaded{};
-   bool pairValid = false;
-   uint32_t *dEpoch = nullptr; // [0] = epoch, [1] = done counter
-   uint32_t hostEpoch = 1;
-   int32_t *dMatch = nullptr;
-   int32_t *hMatch = nullptr; // pinned
-   int *dFlag = nullptr;
-   int *hFlag = nullptr;      // pinned
-
-   std::vector<Mask> masks; // active since the last find
-   Mask *dMasks = nullptr;
-
-   // plain path
-   float *dS = nullptr;
-   size_t sBytes = 0;
-   void *dPart = nullptr;
-   size_t partBytes = 0;
-   int32_t *dColMax = nullptr, *dRowMax = nullptr;
-   bool sValid = false;     // dS holds the scores of the current descriptors
-   int sMasksApplied = 0;   // masks already applied to dS
-   bool lastWasPlain = false;
-
-   bool pendingAsync = false;
-   int lastNL = 0;
-
-   // survivor regions of the fused scan
-   uint64_t *dSurv = nullptr;
-   int32_t *dSurvCount = nullptr;
-   int survCap = 64;
-
-   // kernel timing: three events per job (before scan, after scan, after select)
-   bool profile = false;
-   int profJobs = 0;
-   std::vector<hipEvent_t> events;
-};
-
-namespace {
-
-int fail(v3dm_ctx *c, int code, const char *fmt, ...)
-{
-   char buf[512];
-   va_list ap;
-   va_start(ap, fmt);
-   vsnprintf(buf, sizeof(buf), fmt, ap);
-   va_end(ap);
-   if (c) c->err = buf;
-   else g_createError = buf;
-   return code;
-}
-
-#define HIPCHK(c, call)                                                                                  \
-   do                                                                                                    \
-   {                                                                                                     \
-      hipError_t e_ = (call);                                                                            \
-      if (e_ != hipSuccess)                                                                              \
-         return fail((c), e_ == hipErrorOutOfMemory ? V3DM_ERR_NOMEM : V3DM_ERR_DEVICE, "%s: %s (%s:%d)", \
-                     #call, hipGetErrorString(e_), __FILE__, __LINE__);                                  \
-   } while (0)
-
-template <class T> void dfree(T *&p)
-{
-   if (p) (void)hipFree((void *)p);
-   p = nullptr;
-}
+namespace v3dm_host {
 
 void freeSide(Side &s)
 {
@@ -325,8 +176,8 @@ int uploadPair(v3dm_ctx *c)
    pd.slotR = 0;
    pd.nL = c->L.n;
    pd.nR = c->R.n;
-   pd.tilesM = (roundUp(pd.nL, kRowPad) + scan_tile_m() - 1) / scan_tile_m();
-   pd.tilesN = (roundUp(pd.nR, kRowPad) + scan_tile_n() - 1) / scan_tile_n();
+   pd.tilesM = (roundUp(pd.nL, kRowPad) + scan_variant_tile_m(0) - 1) / scan_variant_tile_m(0);
+   pd.tilesN = (roundUp(pd.nR, kRowPad) + scan_variant_tile_n(0) - 1) / scan_variant_tile_n(0);
    pd.tileBase = 0;
    pd.finBase = 0;
    if (c->pairValid && memcmp(&pd, &c->hPairUploaded, sizeof(pd)) == 0) return V3DM_OK;
@@ -395,8 +246,13 @@ int enqueueI8(v3dm_ctx *c, float minScore, float minRatio)
    a.surv = c->dSurv;
    a.survCount = c->dSurvCount;
    a.survCap = c->survCap;
+   a.waves = scan_variant_waves(0);
+   a.tileM = scan_variant_tile_m(0);
+   a.tileN = scan_variant_tile_n(0);
+   a.persistent = 0;
+   a.tileCounter = c->dEpoch + 2;
    if ((rc = profMark(c, 0))) return rc;
-   HIPCHK(c, launch_i8_scan(a, c->stream));
+   HIPCHK(c, launch_i8_scan(0, a.nTiles, a, c->stream));
    if ((rc = profMark(c, 1))) return rc;
    HIPCHK(c, launch_i8_insert(a, c->stream));
    FinalizeArgs f{};
@@ -409,6 +265,7 @@ int enqueueI8(v3dm_ctx *c, float minScore, float minRatio)
    f.top2R = c->R.dTop2;
    f.epoch = c->dEpoch;
    f.done = c->dEpoch + 1;
+   f.tileCounter = c->dEpoch + 2;
    f.minScore = minScore;
    f.minRatio = minRatio;
    f.matchOfLeft = c->dMatch;
@@ -594,6 +451,7 @@ void v3dm_destroy(v3dm_ctx *c)
    (void)hipSetDevice(c->device);
    if (c->stream) (void)hipStreamSynchronize(c->stream);
    freeAll(c);
+   freeBatch(c);
    for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
    if (c->stream) (void)hipStreamDestroy(c->stream);
    delete c;
@@ -657,16 +515,17 @@ int v3dm_allocate(v3dm_ctx *c, int nMaxLeft, int nMaxRight, int useGuided)
    }
    const int nPadL = std::max(roundUp(nMaxLeft, kRowPad), kRowPad);
    HIPCHK(c, hipMalloc((void **)&c->dPair, sizeof(PairDesc)));
-   HIPCHK(c, hipMalloc((void **)&c->dEpoch, 2 * sizeof(uint32_t)));
+   HIPCHK(c, hipMalloc((void **)&c->dEpoch, 4 * sizeof(uint32_t))); // epoch, done, tile counter
    HIPCHK(c, hipMalloc((void **)&c->dMatch, sizeof(int32_t) * nPadL));
    HIPCHK(c, hipMalloc((void **)&c->dFlag, sizeof(int)));
    HIPCHK(c, hipMalloc((void **)&c->dMasks, sizeof(Mask) * kMaxMasks));
    HIPCHK(c, hipMalloc((void **)&c->dColMax, sizeof(int32_t) * nPadL));
    HIPCHK(c, hipMalloc((void **)&c->dRowMax, sizeof(int32_t) * std::max(roundUp(nMaxRight, kRowPad), kRowPad)));
    {
-      const int64_t tiles = (int64_t)((roundUp(std::max(nMaxLeft, 1), kRowPad) + scan_tile_m() - 1) / scan_tile_m()) *
-                            ((roundUp(std::max(nMaxRight, 1), kRowPad) + scan_tile_n() - 1) / scan_tile_n());
-      const int64_t regions = tiles * scan_waves();
+      const int64_t tiles =
+         (int64_t)((roundUp(std::max(nMaxLeft, 1), kRowPad) + scan_variant_tile_m(0) - 1) / scan_variant_tile_m(0)) *
+         ((roundUp(std::max(nMaxRight, 1), kRowPad) + scan_variant_tile_n(0) - 1) / scan_variant_tile_n(0));
+      const int64_t regions = tiles * scan_variant_waves(0);
       HIPCHK(c, hipMalloc((void **)&c->dSurv, sizeof(uint64_t) * (size_t)regions * c->survCap));
       HIPCHK(c, hipMalloc((void **)&c->dSurvCount, sizeof(int32_t) * (size_t)regions));
    }

@@ -65,6 +65,9 @@ struct ScanArgs
    uint64_t *surv;              // survivor regions, survCap entries per (tile, wave)
    int32_t *survCount;          // entries written per region (may exceed survCap)
    int survCap;
+   int waves, tileM, tileN;     // tile geometry of the launched scan variant
+   int persistent;              // 1: resident workgroups pull tiles from *tileCounter
+   uint32_t *tileCounter;       // reset to 0 by the finalize kernel
 };
 
 struct FinalizeArgs
@@ -75,6 +78,7 @@ struct FinalizeArgs
    const uint64_t *top1L, *top2L, *top1R, *top2R;
    uint32_t *epoch;
    uint32_t *done;
+   uint32_t *tileCounter;       // persistent scan's queue head, reset at the end of the job
    float minScore, minRatio;
    int32_t *matchOfLeft; // per left slot: matched right index or -1
 };

@@ -116,6 +116,63 @@ class SIFTMatching:
         check(self._lib.v3dm_profile_read(self._ctx, ctypes.byref(a), ctypes.byref(b), ctypes.byref(n)), self._ctx)
         return a.value, b.value, n.value
 
+    # -- many pairs per job (v3dm_batch_*) ---------------------------------------
+    def setImages(self, images) -> list[int]:
+        """Uploads uint8 descriptor sets (list of (n_i, 128) arrays); returns
+        the counts after rounding to multiples of 32."""
+        counts = np.array([len(x) for x in images], np.int32)
+        desc = np.ascontiguousarray(np.concatenate([np.asarray(x, np.uint8).reshape(-1, 128) for x in images])
+                                    if len(images) else np.zeros((0, 128), np.uint8))
+        check(self._lib.v3dm_batch_set_images_u8(self._ctx, len(images), ptr(counts, ctypes.c_int32),
+                                                 ptr(desc, ctypes.c_uint8)), self._ctx)
+        self._img_counts = [int(c) - int(c) % 32 for c in counts]
+        return self._img_counts
+
+    def setPairs(self, pairs) -> None:
+        pairs = np.ascontiguousarray(np.asarray(pairs, np.int32).reshape(-1, 2))
+        check(self._lib.v3dm_batch_set_pairs(self._ctx, len(pairs), ptr(pairs, ctypes.c_int32)), self._ctx)
+        self._pairs = pairs
+
+    def batchInfo(self) -> dict:
+        n, nl, nt, ps = ctypes.c_int(), ctypes.c_int64(), ctypes.c_int(), ctypes.c_int64()
+        check(self._lib.v3dm_batch_info(self._ctx, ctypes.byref(n), ctypes.byref(nl), ctypes.byref(nt),
+                                        ctypes.byref(ps)), self._ctx)
+        return {"pairs": n.value, "left_total": nl.value, "tiles": nt.value, "pairs_scored": ps.value}
+
+    def batchMatchAsync(self, minScore: float = 0.9, minRatio: float = 0.8) -> None:
+        check(self._lib.v3dm_batch_match_async(self._ctx, minScore, minRatio), self._ctx)
+
+    def batchFetch(self) -> list[np.ndarray]:
+        info = self.batchInfo()
+        tab = np.empty(max(info["left_total"], 1), np.int32)
+        off = np.empty(info["pairs"] + 1, np.int64)
+        check(self._lib.v3dm_batch_fetch(self._ctx, ptr(tab, ctypes.c_int32), ptr(off, ctypes.c_int64)), self._ctx)
+        return [tab[off[p]:off[p + 1]].copy() for p in range(info["pairs"])]
+
+    def batchMatch(self, minScore: float = 0.9, minRatio: float = 0.8) -> list[np.ndarray]:
+        """Per pair: table (right index or -1 per left feature)."""
+        self.batchMatchAsync(minScore, minRatio)
+        return self.batchFetch()
+
+    def batchSetVariant(self, variant: int) -> None:
+        check(self._lib.v3dm_batch_set_variant(self._ctx, int(variant)), self._ctx)
+
+    def batchVariant(self) -> int:
+        v = ctypes.c_int()
+        check(self._lib.v3dm_batch_get_variant(self._ctx, ctypes.byref(v)), self._ctx)
+        return v.value
+
+    def batchAutotune(self, minScore: float = 0.9, minRatio: float = 0.8, reps: int = 3):
+        n = self._lib.v3dm_scan_variant_count()
+        ms = np.zeros(n, np.float64)
+        best = ctypes.c_int()
+        check(self._lib.v3dm_batch_autotune(self._ctx, minScore, minRatio, reps, ptr(ms, ctypes.c_double),
+                                            ctypes.byref(best)), self._ctx)
+        return best.value, ms
+
+    def batchSetGraph(self, enable: bool = True) -> None:
+        check(self._lib.v3dm_batch_set_graph(self._ctx, int(enable)), self._ctx)
+
     def synchronize(self) -> None:
         check(self._lib.v3dm_synchronize(self._ctx), self._ctx)
 
