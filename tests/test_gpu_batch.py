@@ -98,6 +98,28 @@ def test_batch_persistent_queue(m, gpu_available):
             np.testing.assert_array_equal(got[p], want[p], err_msg=f"rep {rep} pair {p}")
 
 
+def test_batch_pool_growth(gpu_available):
+    """Image pools that grow and shrink between jobs: every pool buffer
+    (descriptors and their row sums) must follow the new size."""
+    mm = v.SIFTMatching(0)
+    mm.enableRoundingWarning(False)
+    try:
+        for seed, counts in [(21, [300, 640]), (22, [2048, 4096, 3072]), (23, [96, 512])]:
+            say("pool growth", counts)
+            imgs = make_images(seed, counts)
+            c32 = mm.setImages(imgs)
+            pairs = [(i, i + 1) for i in range(len(counts) - 1)] + [(len(counts) - 1, 0)]
+            mm.setPairs(pairs)
+            for variant in (0, 3):
+                mm.batchSetVariant(variant)
+                got = mm.batchMatch(0.8, 0.8)
+                want = expected_tables(imgs, c32, pairs, 0.8, 0.8)
+                for p in range(len(pairs)):
+                    np.testing.assert_array_equal(got[p], want[p], err_msg=f"{counts} variant {variant} pair {p}")
+    finally:
+        mm.close()
+
+
 def test_batch_graph_replays(m, batch):
     """Graph replays interleaved with eager jobs and other thresholds."""
     imgs, c32, pairs = batch

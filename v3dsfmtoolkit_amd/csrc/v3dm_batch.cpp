@@ -28,7 +28,8 @@ struct BatchState
    int64_t poolRows = 0;
    int8_t *dI8 = nullptr;
    int32_t *dSum = nullptr;
-   int64_t poolCap = 0;
+   int64_t poolCap = 0; // bytes of dI8
+   int64_t sumCap = 0;  // rows of dSum
    uint8_t *dRaw = nullptr;
    int64_t rawCap = 0;
 
@@ -432,12 +433,8 @@ int v3dm_batch_set_images_u8(v3dm_ctx *c, int nImages, const int32_t *counts, co
    if (total > 0 && !desc) return fail(c, V3DM_ERR_ARG, "batch_set_images: descriptor pointer is NULL");
    int rc = growDevice(c, b->dI8, b->poolCap, rows * kDim);
    if (rc) return rc;
-   int64_t sumCap = b->poolCap / kDim;
-   if (!b->dSum || rows > sumCap)
-   {
-      dfree(b->dSum);
-      HIPCHK(c, hipMalloc((void **)&b->dSum, sizeof(int32_t) * (size_t)std::max<int64_t>(rows, 1)));
-   }
+   rc = growDevice(c, b->dSum, b->sumCap, rows);
+   if (rc) return rc;
    rc = growDevice(c, b->dRaw, b->rawCap, total * kDim);
    if (rc) return rc;
    if (total > 0) HIPCHK(c, hipMemcpyAsync(b->dRaw, desc, (size_t)total * kDim, hipMemcpyHostToDevice, c->stream));
