@@ -45,6 +45,32 @@ def cpu_baseline(L, R, min_score, min_ratio, budget_s=20.0):
             "seconds_per_pair": med}, tab
 
 
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_i8_scan_8192x8192_traffic.md")
+
+
+def measured_traffic(kernel_name):
+    """HBM bytes per launch of `kernel_name` at the default workload, as the
+    committed rocprofv3 counter summary states it (a line
+    `traffic_bytes_per_launch <kernel> = <bytes>`), or None."""
+    try:
+        with open(TRAFFIC_SUMMARY) as f:
+            for line in f:
+                line = line.strip().strip("`")
+                if line.startswith(f"traffic_bytes_per_launch {kernel_name} ="):
+                    return float(line.split("=")[1])
+    except OSError:
+        return None
+    return None
+
+
+def copy_probe(mib, reps=5):
+    """A plain device copy of known size (k_copy_probe: reads and writes
+    exactly `mib` MiB per launch), run in the same counter pass as the matcher
+    so FETCH_SIZE / WRITE_SIZE readings can be scaled to bytes."""
+    import v3dsfmtoolkit_amd as v
+    v.copy_probe(mib << 20, reps)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -54,6 +80,8 @@ def main():
     ap.add_argument("--n", type=int, default=8192)
     ap.add_argument("--plain-steps", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="also run a plain device copy of this size (scales memory counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -119,7 +147,8 @@ def main():
     if kernel == v.KERNEL_I8:
         achieved = pairs * 256 / scan_avg_s / 1e12
         roof = {"bound": "mfma", "kernel": "k_i8_scan", "achieved": achieved, "peak": INT8_PEAK_TOPS,
-                "unit": "TFLOP/s", "frac": achieved / INT8_PEAK_TOPS, "traffic": None,
+                "unit": "TFLOP/s", "frac": achieved / INT8_PEAK_TOPS,
+                "traffic": measured_traffic("k_i8_scan") if n == 8192 else None,
                 "ops_per_pair": 256, "launch_avg_us": scan_avg_s * 1e6}
     else:
         achieved = pairs * 256 / scan_avg_s / 1e12
@@ -158,6 +187,8 @@ def main():
         cb["agrees_with_gpu"] = O.table_to_pairs(tab) == got
         out["cpu_baseline"] = cb
     m.close()
+    if args.copy_probe_mib > 0:
+        copy_probe(args.copy_probe_mib)
     print(json.dumps(out), flush=True)
 
 

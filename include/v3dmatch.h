@@ -217,6 +217,13 @@ int v3dm_device_count(int *count);
  * register v of lane l holds row 8*(v/4) + 4*(l/32) + v%4, column l%32. */
 int v3dm_mfma_layout_probe(int device, int32_t *out);
 
+/* Diagnostic, no counterpart in the reference: copies `bytes` (a multiple of
+ * 16) between two fresh device buffers `reps` times with a plain coalesced
+ * copy kernel (k_copy_probe), each launch reading and writing exactly `bytes`.
+ * Memory-counter passes (FETCH_SIZE, WRITE_SIZE) run it beside the matcher to
+ * scale their readings to bytes. */
+int v3dm_copy_probe(int device, int64_t bytes, int reps);
+
 #ifdef __cplusplus
 }
 #endif

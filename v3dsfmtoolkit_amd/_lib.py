@@ -80,6 +80,7 @@ _SIGNATURES = {
     "v3dm_synth_seed": (ctypes.c_uint64, [ctypes.c_int, ctypes.c_int]),
     "v3dm_device_count": (ctypes.c_int, [_c_int_p]),
     "v3dm_mfma_layout_probe": (ctypes.c_int, [ctypes.c_int, _i32_p]),
+    "v3dm_copy_probe": (ctypes.c_int, [ctypes.c_int, ctypes.c_int64, ctypes.c_int]),
 }
 
 _lib = None
@@ -162,6 +163,11 @@ def mfma_layout_probe(device: int = 0) -> np.ndarray:
     out = np.zeros((64, 16), np.int32)
     check(lib().v3dm_mfma_layout_probe(device, ptr(out, ctypes.c_int32)))
     return out
+
+
+def copy_probe(nbytes: int, reps: int = 5, device: int = 0) -> None:
+    """Plain device copy of nbytes (read once, written once) per launch."""
+    check(lib().v3dm_copy_probe(device, nbytes, reps))
 
 
 def scan_variants() -> list[tuple[int, int, int]]:

@@ -39,6 +39,8 @@
 // pass is needed between jobs.
 #include <type_traits>
 
+#include <hip/hip_ext.h>
+
 #include "v3dm_internal.h"
 
 namespace v3dm {
@@ -555,15 +557,17 @@ int scan_variant_resident(int v, int device)
    return (perCU > 0 ? perCU : 1) * (cus > 0 ? cus : 1);
 }
 
-hipError_t launch_i8_scan(int variant, int grid, const ScanArgs &a, hipStream_t st)
+hipError_t launch_i8_scan(int variant, int grid, const ScanArgs &a, hipStream_t st, hipEvent_t start,
+                          hipEvent_t stop)
 {
    if (a.nTiles <= 0) return hipSuccess;
+   // start / stop (may be null) take the timestamps of this dispatch itself
    switch (variant)
    {
-   case 0: hipLaunchKernelGGL((k_i8_scan<4, 512>), dim3(grid), dim3(256), 0, st, a); break;
-   case 1: hipLaunchKernelGGL((k_i8_scan<4, 1024>), dim3(grid), dim3(256), 0, st, a); break;
-   case 2: hipLaunchKernelGGL((k_i8_scan<8, 512>), dim3(grid), dim3(512), 0, st, a); break;
-   case 3: hipLaunchKernelGGL((k_i8_scan<4, 256>), dim3(grid), dim3(256), 0, st, a); break;
+   case 0: hipExtLaunchKernelGGL((k_i8_scan<4, 512>), dim3(grid), dim3(256), 0, st, start, stop, 0, a); break;
+   case 1: hipExtLaunchKernelGGL((k_i8_scan<4, 1024>), dim3(grid), dim3(256), 0, st, start, stop, 0, a); break;
+   case 2: hipExtLaunchKernelGGL((k_i8_scan<8, 512>), dim3(grid), dim3(512), 0, st, start, stop, 0, a); break;
+   case 3: hipExtLaunchKernelGGL((k_i8_scan<4, 256>), dim3(grid), dim3(256), 0, st, start, stop, 0, a); break;
    default: return hipErrorInvalidValue;
    }
    return hipGetLastError();

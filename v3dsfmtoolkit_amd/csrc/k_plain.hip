@@ -13,6 +13,8 @@
 // the exact integer score times 2^-18 whatever the order.  Selection states
 // (max, next, arg) are merged with the first-index rule, which equals the
 // reference's sequential scan for any split of the index range.
+#include <hip/hip_ext.h>
+
 #include "v3dm_internal.h"
 
 namespace v3dm {
@@ -304,13 +306,22 @@ __global__ void k_u8_to_f32(const uint8_t *__restrict__ src, int64_t count, floa
    if (i < count) dst[i] = (float)src[i] * (1.0f / 512.0f);
 }
 
+// Memory-counter calibration: copies n16 16-byte units, each read once and
+// written once (grid-stride, fully coalesced).
+__global__ __launch_bounds__(256) void k_copy_probe(const uint4 *__restrict__ src, uint4 *__restrict__ dst, int64_t n16)
+{
+   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+}
+
 // ---- launchers ------------------------------------------------------------
 
-hipError_t launch_sgemm_seq(int nL, int nR, const float *Lf, const float *Rf, float *S, hipStream_t st)
+hipError_t launch_sgemm_seq(int nL, int nR, const float *Lf, const float *Rf, float *S, hipStream_t st,
+                            hipEvent_t start, hipEvent_t stop)
 {
    if (nL <= 0 || nR <= 0) return hipSuccess;
    dim3 grid((nL + kGemmT - 1) / kGemmT, (nR + kGemmT - 1) / kGemmT);
-   hipLaunchKernelGGL(k_sgemm_seq, grid, dim3(256), 0, st, nL, nR, Lf, Rf, S);
+   hipExtLaunchKernelGGL(k_sgemm_seq, grid, dim3(256), 0, st, start, stop, 0, nL, nR, Lf, Rf, S);
    return hipGetLastError();
 }
 
@@ -378,6 +389,14 @@ hipError_t launch_u8_to_f32(const uint8_t *src, int64_t count, float *dst, hipSt
 {
    if (count <= 0) return hipSuccess;
    hipLaunchKernelGGL(k_u8_to_f32, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, src, count, dst);
+   return hipGetLastError();
+}
+
+hipError_t launch_copy_probe(const void *src, void *dst, int64_t bytes, hipStream_t st)
+{
+   const int64_t n16 = bytes / 16;
+   if (n16 <= 0) return hipSuccess;
+   hipLaunchKernelGGL(k_copy_probe, dim3(2048), dim3(256), 0, st, (const uint4 *)src, (uint4 *)dst, n16);
    return hipGetLastError();
 }
 

@@ -26,19 +26,41 @@ void freeSide(Side &s)
 
 constexpr int kMaxProfiledJobs = 65536;
 
-// Records event `which` (0, 1, 2) of the current profiled job.
-int profMark(v3dm_ctx *c, int which)
+// Events of a profiled job: 0 and 1 bracket the score kernel, 2 follows the
+// selection.  Creates the events of the current job when needed.
+static int profEnsure(v3dm_ctx *c)
 {
-   if (!c->profile || c->profJobs >= kMaxProfiledJobs) return V3DM_OK;
-   const size_t idx = (size_t)c->profJobs * 3 + which;
-   while (c->events.size() <= idx)
+   const size_t need = (size_t)c->profJobs * 3 + 3;
+   while (c->events.size() < need)
    {
       hipEvent_t e;
       HIPCHK(c, hipEventCreate(&e));
       c->events.push_back(e);
    }
-   HIPCHK(c, hipEventRecord(c->events[idx], c->stream));
+   return V3DM_OK;
+}
+
+// Records event `which` (0, 1, 2) of the current profiled job.
+int profMark(v3dm_ctx *c, int which)
+{
+   if (!c->profile || c->profJobs >= kMaxProfiledJobs) return V3DM_OK;
+   int rc = profEnsure(c);
+   if (rc) return rc;
+   HIPCHK(c, hipEventRecord(c->events[(size_t)c->profJobs * 3 + which], c->stream));
    if (which == 2) c->profJobs++;
+   return V3DM_OK;
+}
+
+// Events 0 and 1 of the current profiled job, to be bound to the score
+// kernel's dispatch (its own start and end timestamps); null when not profiling.
+int profKernelEvents(v3dm_ctx *c, hipEvent_t *start, hipEvent_t *stop)
+{
+   *start = *stop = nullptr;
+   if (!c->profile || c->profJobs >= kMaxProfiledJobs) return V3DM_OK;
+   int rc = profEnsure(c);
+   if (rc) return rc;
+   *start = c->events[(size_t)c->profJobs * 3];
+   *stop = c->events[(size_t)c->profJobs * 3 + 1];
    return V3DM_OK;
 }
 
@@ -251,9 +273,14 @@ int enqueueI8(v3dm_ctx *c, float minScore, float minRatio)
    a.tileN = scan_variant_tile_n(0);
    a.persistent = 0;
    a.tileCounter = c->dEpoch + 2;
-   if ((rc = profMark(c, 0))) return rc;
-   HIPCHK(c, launch_i8_scan(0, a.nTiles, a, c->stream));
-   if ((rc = profMark(c, 1))) return rc;
+   hipEvent_t e0, e1;
+   if ((rc = profKernelEvents(c, &e0, &e1))) return rc;
+   if (a.nTiles <= 0 && e0)
+   {
+      HIPCHK(c, hipEventRecord(e0, c->stream));
+      HIPCHK(c, hipEventRecord(e1, c->stream));
+   }
+   HIPCHK(c, launch_i8_scan(0, a.nTiles, a, c->stream, e0, e1));
    HIPCHK(c, launch_i8_insert(a, c->stream));
    FinalizeArgs f{};
    f.pairs = c->dPair;
@@ -306,14 +333,19 @@ int enqueuePlain(v3dm_ctx *c, float minScore, float minRatio)
       HIPCHK(c, hipMalloc(&c->dPart, pNeed));
       c->partBytes = pNeed;
    }
-   if ((rc = profMark(c, 0))) return rc;
+   hipEvent_t e0, e1;
+   if ((rc = profKernelEvents(c, &e0, &e1))) return rc;
    if (!c->sValid)
    {
-      HIPCHK(c, launch_sgemm_seq(nL, nR, c->L.dF32, c->R.dF32, c->dS, c->stream));
+      HIPCHK(c, launch_sgemm_seq(nL, nR, c->L.dF32, c->R.dF32, c->dS, c->stream, e0, e1));
       c->sValid = true;
       c->sMasksApplied = 0;
    }
-   if ((rc = profMark(c, 1))) return rc;
+   else if (e0)
+   {
+      HIPCHK(c, hipEventRecord(e0, c->stream));
+      HIPCHK(c, hipEventRecord(e1, c->stream));
+   }
    const int nm = (int)c->masks.size();
    if (nm > c->sMasksApplied)
    {
@@ -694,6 +726,25 @@ int v3dm_mfma_layout_probe(int device, int32_t *out)
    if (e == hipSuccess) e = hipMemcpy(out, d, sizeof(int32_t) * 1024, hipMemcpyDeviceToHost);
    if (d) (void)hipFree(d);
    if (e != hipSuccess) return fail(nullptr, V3DM_ERR_DEVICE, "mfma_layout_probe: %s", hipGetErrorString(e));
+   return V3DM_OK;
+}
+
+int v3dm_copy_probe(int device, int64_t bytes, int reps)
+{
+   if (bytes <= 0 || bytes % 16 || reps < 1) return fail(nullptr, V3DM_ERR_ARG, "copy_probe: bad size or count");
+   int n = 0;
+   if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n)
+      return fail(nullptr, V3DM_ERR_DEVICE, "copy_probe: device %d not available", device);
+   hipError_t e = hipSetDevice(device);
+   void *src = nullptr, *dst = nullptr;
+   if (e == hipSuccess) e = hipMalloc(&src, (size_t)bytes);
+   if (e == hipSuccess) e = hipMalloc(&dst, (size_t)bytes);
+   if (e == hipSuccess) e = hipMemset(src, 1, (size_t)bytes);
+   for (int r = 0; r < reps && e == hipSuccess; ++r) e = launch_copy_probe(src, dst, bytes, nullptr);
+   if (e == hipSuccess) e = hipDeviceSynchronize();
+   if (src) (void)hipFree(src);
+   if (dst) (void)hipFree(dst);
+   if (e != hipSuccess) return fail(nullptr, V3DM_ERR_DEVICE, "copy_probe: %s", hipGetErrorString(e));
    return V3DM_OK;
 }
 

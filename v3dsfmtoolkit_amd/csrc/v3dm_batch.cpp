@@ -217,9 +217,14 @@ int enqueueKernels(v3dm_ctx *c, BatchState *b, float minScore, float minRatio, b
    int grid = 0;
    fillArgs(c, b, minScore, minRatio, a, f, grid);
    int rc;
-   if (marks && (rc = profMark(c, 0))) return rc;
-   HIPCHK(c, launch_i8_scan(b->variant, grid, a, c->stream));
-   if (marks && (rc = profMark(c, 1))) return rc;
+   hipEvent_t e0 = nullptr, e1 = nullptr;
+   if (marks && (rc = profKernelEvents(c, &e0, &e1))) return rc;
+   if (a.nTiles <= 0 && e0)
+   {
+      HIPCHK(c, hipEventRecord(e0, c->stream));
+      HIPCHK(c, hipEventRecord(e1, c->stream));
+   }
+   HIPCHK(c, launch_i8_scan(b->variant, grid, a, c->stream, e0, e1));
    HIPCHK(c, launch_i8_insert(a, c->stream));
    HIPCHK(c, launch_i8_finalize(f, c->stream));
    if (marks && (rc = profMark(c, 2))) return rc;

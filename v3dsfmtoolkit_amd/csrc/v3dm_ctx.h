@@ -14,7 +14,8 @@
 #include "v3dm_internal.h"
 
 namespace v3dm {
-hipError_t launch_i8_scan(int variant, int grid, const ScanArgs &a, hipStream_t st);
+hipError_t launch_i8_scan(int variant, int grid, const ScanArgs &a, hipStream_t st, hipEvent_t start = nullptr,
+                          hipEvent_t stop = nullptr);
 int scan_variant_count();
 int scan_variant_waves(int v);
 int scan_variant_tile_m(int v);
@@ -24,7 +25,9 @@ hipError_t launch_i8_finalize(const FinalizeArgs &a, hipStream_t st);
 hipError_t launch_i8_insert(const ScanArgs &a, hipStream_t st);
 hipError_t launch_clear_keys(uint64_t *keys, int64_t n, uint32_t *epoch, uint32_t *done, hipStream_t st);
 hipError_t launch_mfma_probe(int32_t *out, hipStream_t st);
-hipError_t launch_sgemm_seq(int nL, int nR, const float *Lf, const float *Rf, float *S, hipStream_t st);
+hipError_t launch_copy_probe(const void *src, void *dst, int64_t bytes, hipStream_t st);
+hipError_t launch_sgemm_seq(int nL, int nR, const float *Lf, const float *Rf, float *S, hipStream_t st,
+                            hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 hipError_t launch_mask_plain(int nL, int nR, float *S, const Mask *masks, int nMasks, hipStream_t st);
 hipError_t launch_fill_i32(int n, int32_t value, int32_t *out, hipStream_t st);
 size_t plain_part_bytes(int nL, int nR, int *nChunksOut, int *rowsPerOut);
@@ -171,6 +174,7 @@ template <class T> inline void dfree(T *&p)
 
 int syncStream(v3dm_ctx *c);
 int profMark(v3dm_ctx *c, int which);
+int profKernelEvents(v3dm_ctx *c, hipEvent_t *start, hipEvent_t *stop);
 void freeBatch(v3dm_ctx *c);
 
 } // namespace v3dm_host
