@@ -38,9 +38,12 @@ extern "C" {
 #define V3DM_ERR_CAPACITY (-6)    /* caller's output buffer too small; *count holds the size needed */
 
 /* kernel selection for v3dm_set_kernel() */
-#define V3DM_KERNEL_AUTO 0  /* int8 MFMA when both sides are u8-representable, else plain FP32 */
+#define V3DM_KERNEL_AUTO 0  /* int8 MFMA when both sides are u8-representable, else BF16 + FP32 rescoring */
 #define V3DM_KERNEL_PLAIN 1 /* reference design: FP32 score matrix in HBM + column/row passes */
 #define V3DM_KERNEL_I8 2    /* fused int8 MFMA matcher (bit-exact for u8 / u-over-512 inputs) */
+#define V3DM_KERNEL_BF16 3  /* float descriptors: BF16 MFMA candidates within a proven error bound of the
+                               decision threshold, rescored with the plain kernel's FP32 arithmetic
+                               (same match list as V3DM_KERNEL_PLAIN) */
 
 typedef struct v3dm_ctx v3dm_ctx;
 
@@ -216,6 +219,14 @@ int v3dm_device_count(int *count);
  * out.  Tests use it to check the accumulator layout the matcher assumes:
  * register v of lane l holds row 8*(v/4) + 4*(l/32) + v%4, column l%32. */
 int v3dm_mfma_layout_probe(int device, int32_t *out);
+
+/* Diagnostic, no counterpart in the reference: per left feature k < n of the
+ * last completed match job, the best and second-best score among the
+ * candidates that reached the exact top-2 (scores above the survivor
+ * threshold; -1e30 where there is none) and the right index of the best (-1
+ * where none).  Exact integer-derived FP32 scores on the int8 path, rescored
+ * FP32 scores on the BF16 path; not available after a plain-kernel job. */
+int v3dm_fetch_left_top2(v3dm_ctx *ctx, int n, float *best, float *second, int32_t *argBest);
 
 /* Diagnostic, no counterpart in the reference: copies `bytes` (a multiple of
  * 16) between two fresh device buffers `reps` times with a plain coalesced

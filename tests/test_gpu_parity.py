@@ -145,12 +145,15 @@ def test_float_inputs_dispatch(m):
     got = run(m, Lq, Rq, 0.8, 0.8, v.KERNEL_AUTO)
     assert m.lastKernel() == v.KERNEL_I8
     assert got == oracle_pairs(d["L"][:672], d["R"][:896], 0.8, 0.8)
-    # unquantised floats: plain kernel, bit-exact with the oracle's sequential-FMA scores
+    # unquantised floats: BF16 candidates + FP32 rescoring, bit-exact with the
+    # oracle's sequential-FMA scores; the plain kernel stays selectable
     Lf, Rf = d["Lf"][:672], d["Rf"][:896]
     for rule in [(0.8, 0.8), (0.9, 0.8), (0.8, -1.0)]:
+        want = O.table_to_pairs(O.find_f32(Lf, Rf, *rule)[2])
         got = run(m, Lf, Rf, *rule, v.KERNEL_AUTO)
-        assert m.lastKernel() == v.KERNEL_PLAIN
-        assert got == O.table_to_pairs(O.find_f32(Lf, Rf, *rule)[2])
+        assert m.lastKernel() == v.KERNEL_BF16
+        assert got == want
+        assert run(m, Lf, Rf, *rule, v.KERNEL_PLAIN) == want
     m.setKernel(v.KERNEL_I8)
     with pytest.raises(v.V3DMError):
         m.findPutativeMatches(0.8, 0.8)

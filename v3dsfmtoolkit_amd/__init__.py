@@ -3,9 +3,9 @@
 The compute runs in libv3dmatch.so (hand-written HIP kernels for gfx950
 behind a C ABI, include/v3dmatch.h); this package is its Python binding.
 """
-from ._lib import (KERNEL_AUTO, KERNEL_I8, KERNEL_PLAIN, V3DMError, copy_probe, device_count, lib,
+from ._lib import (KERNEL_AUTO, KERNEL_BF16, KERNEL_I8, KERNEL_PLAIN, V3DMError, copy_probe, device_count, lib,
                    mfma_layout_probe, scan_variants, synth_pair, synth_seed)
 from .matching import SIFTMatching
 
-__all__ = ["SIFTMatching", "V3DMError", "KERNEL_AUTO", "KERNEL_PLAIN", "KERNEL_I8", "device_count", "lib",
+__all__ = ["SIFTMatching", "V3DMError", "KERNEL_AUTO", "KERNEL_PLAIN", "KERNEL_I8", "KERNEL_BF16", "device_count", "lib",
            "mfma_layout_probe", "copy_probe", "scan_variants", "synth_pair", "synth_seed"]

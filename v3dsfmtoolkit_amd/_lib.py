@@ -27,6 +27,7 @@ V3DM_ERR_CAPACITY = -6
 KERNEL_AUTO = 0
 KERNEL_PLAIN = 1
 KERNEL_I8 = 2
+KERNEL_BF16 = 3
 
 _c_int_p = ctypes.POINTER(ctypes.c_int)
 _i32_p = ctypes.POINTER(ctypes.c_int32)
@@ -81,6 +82,7 @@ _SIGNATURES = {
     "v3dm_device_count": (ctypes.c_int, [_c_int_p]),
     "v3dm_mfma_layout_probe": (ctypes.c_int, [ctypes.c_int, _i32_p]),
     "v3dm_copy_probe": (ctypes.c_int, [ctypes.c_int, ctypes.c_int64, ctypes.c_int]),
+    "v3dm_fetch_left_top2": (ctypes.c_int, [_ctx_p, ctypes.c_int, _f32_p, _f32_p, _i32_p]),
 }
 
 _lib = None

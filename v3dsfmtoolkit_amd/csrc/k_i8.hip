@@ -41,73 +41,17 @@
 
 #include <hip/hip_ext.h>
 
-#include "v3dm_internal.h"
+#include "v3dm_device.h"
 
 namespace v3dm {
 
-typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 
-constexpr int kWaveRows = 64;       // left rows per wave (two 32-row MFMA blocks)
 constexpr int kBias = 1 << 21;      // 128^3
 
-__device__ inline int find_pair_by_tile(const PairDesc *pairs, int nPairs, int tile)
-{
-   int lo = 0, hi = nPairs - 1;
-   while (lo < hi)
-   {
-      int mid = (lo + hi + 1) >> 1;
-      if (pairs[mid].tileBase <= tile) lo = mid;
-      else hi = mid - 1;
-   }
-   return lo;
-}
 
-__device__ inline int find_pair_by_fin(const PairDesc *pairs, int nPairs, int blk)
-{
-   int lo = 0, hi = nPairs - 1;
-   while (lo < hi)
-   {
-      int mid = (lo + hi + 1) >> 1;
-      if (pairs[mid].finBase <= blk) lo = mid;
-      else hi = mid - 1;
-   }
-   return lo;
-}
 
-// Epipolar Sampson test, v3d_cudamatching.cu:266-290 (this file is compiled
-// with -ffp-contract=off so the expression is evaluated as written).
-__device__ inline bool mask_hit(const Mask &mk, float px, float py, float qx, float qy)
-{
-   const float *F = mk.m;
-   if (mk.kind == 1)
-   {
-      float Fpx = F[0] * px + F[1] * py + F[2];
-      float Fpy = F[3] * px + F[4] * py + F[5];
-      float Fpz = F[6] * px + F[7] * py + F[8];
-      float Ftqx = F[0] * qx + F[3] * qy + F[6];
-      float Ftqy = F[1] * qx + F[4] * qy + F[7];
-      float num = qx * Fpx + qy * Fpy + Fpz;
-      num *= num;
-      float denom = Fpx * Fpx + Fpy * Fpy + Ftqx * Ftqx + Ftqy * Ftqy;
-      float sampson = num / denom;
-      return sampson > mk.sqrThr;
-   }
-   // homography transfer, v3d_cudamatching.cu:306-322
-   float Hpx = F[0] * px + F[1] * py + F[2];
-   float Hpy = F[3] * px + F[4] * py + F[5];
-   float Hpz = F[6] * px + F[7] * py + F[8];
-   Hpx /= Hpz;
-   Hpy /= Hpz;
-   float sqrDist = (Hpx - qx) * (Hpx - qx) + (Hpy - qy) * (Hpy - qy);
-   return sqrDist > mk.sqrThr;
-}
 
-__device__ inline void top2_insert(uint64_t *t1, uint64_t *t2, uint64_t key)
-{
-   unsigned long long old = atomicMax((unsigned long long *)t1, (unsigned long long)key);
-   atomicMax((unsigned long long *)t2, (unsigned long long)(old < key ? old : key));
-}
 
 __device__ __forceinline__ void survivor(const ScanArgs &a, const PairDesc &pd, uint32_t epoch, int il, int jr,
                                          int32_t score)
@@ -122,10 +66,6 @@ __device__ __forceinline__ void survivor(const ScanArgs &a, const PairDesc &pd, 
    top2_insert(a.top1R + pd.slotR + jr, a.top2R + pd.slotR + jr, pack_key(epoch, (uint32_t)score, (uint32_t)il));
 }
 
-__device__ inline v4i ld16(const int8_t *p)
-{
-   return *(__attribute__((address_space(1))) const v4i *)(p);
-}
 
 __device__ inline int max16(const v16i &x)
 {
@@ -145,7 +85,6 @@ __device__ inline int select32(const v16i &a0, const v16i &a1, int v)
    return v < 16 ? x0 : x1;
 }
 
-__device__ inline int sgpr(int x) { return __builtin_amdgcn_readfirstlane(x); }
 
 // Operand packing.  Descriptor byte e of a row goes to lane half h = e / 64,
 // MFMA step s = (e % 64) / 16, byte e % 16 of the lane's 16-byte operand.
@@ -169,21 +108,11 @@ __device__ inline int sgpr(int x) { return __builtin_amdgcn_readfirstlane(x); }
 // swz(r) = (r ^ r >> 3) & 7, so the 32 rows of one fragment read (same
 // chunk) fall in distinct bank groups for 8- and 16-lane LDS service.
 
-constexpr int kStepCols = 32;                      // right rows per step
 constexpr int kStageBytes = kStepCols * kDim;      // 4 KiB
 constexpr int kGroup = 4;                          // steps per barrier
 constexpr int kGroupBytes = kGroup * kStageBytes;  // 16 KiB
-constexpr int kThirds = 3;
-constexpr int kRsrcWord3 = 0x00020000;             // raw buffer, 32-bit data format
 
-__device__ inline int swz(int r) { return (r ^ (r >> 3)) & 7; }
 
-__device__ inline void lds_barrier()
-{
-   // LDS writes of this wave done, then the workgroup barrier.  Written out
-   // so no vmcnt wait is attached: the register ring stays in flight.
-   asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
-}
 
 // One workgroup per tile, or (a.persistent) resident workgroups that take
 // tiles blockIdx.x, then gridDim.x + atomicAdd(tileCounter) until the tiles
@@ -437,15 +366,6 @@ __global__ __launch_bounds__(256) void k_i8_insert(ScanArgs a)
    }
 }
 
-__device__ inline bool rule_accept(float best, float next, float minScore, float minRatio)
-{
-   // v3d_cudamatching.cu:32 (ratio test off) and :138-145 / :187-193
-   if (minRatio < 0) return best > minScore;
-   float nextDist = fmaxf(0.001f, 2.0f - 2.0f * next);
-   float bestDist = 2.0f - 2.0f * best;
-   float ratio = bestDist / nextDist;
-   return best > minScore && ratio < minRatio;
-}
 
 __device__ inline float key_value(uint64_t k, uint32_t epoch)
 {

@@ -21,6 +21,7 @@ void freeSide(Side &s)
    dfree(s.dTop1);
    dfree(s.dTop2);
    dfree(s.dPos);
+   dfree(s.dBF);
    s = Side();
 }
 
@@ -79,6 +80,7 @@ void freeAll(v3dm_ctx *c)
    dfree(c->dRowMax);
    dfree(c->dSurv);
    dfree(c->dSurvCount);
+   dfree(c->dNorm);
    if (c->hMatch) (void)hipHostFree(c->hMatch);
    if (c->hFlag) (void)hipHostFree(c->hFlag);
    c->hMatch = nullptr;
@@ -88,6 +90,7 @@ void freeAll(v3dm_ctx *c)
    c->pairValid = false;
    c->sValid = false;
    c->masks.clear();
+   c->keyFormat = 0;
 }
 
 int allocSide(v3dm_ctx *c, Side &s, int nMax, bool guided)
@@ -111,6 +114,20 @@ int clearKeys(v3dm_ctx *c)
    HIPCHK(c, launch_clear_keys(c->R.dTop1, nPadR, c->dEpoch, c->dEpoch + 1, c->stream));
    HIPCHK(c, launch_clear_keys(c->R.dTop2, nPadR, c->dEpoch, c->dEpoch + 1, c->stream));
    c->hostEpoch = 1;
+   return V3DM_OK;
+}
+
+// The top-2 arrays hold keys of one format; a job of the other format
+// starts from cleared arrays.
+int useKeyFormat(v3dm_ctx *c, int format)
+{
+   const uint32_t limit = format == 1 ? kEpochLimitF : kEpochLimit;
+   if (c->keyFormat != format || c->hostEpoch >= limit)
+   {
+      int rc = clearKeys(c);
+      if (rc) return rc;
+      c->keyFormat = format;
+   }
    return V3DM_OK;
 }
 
@@ -152,6 +169,7 @@ int setFeatures(v3dm_ctx *c, Side &s, int n, const void *desc, bool u8, const ch
    const int nPad = std::max(roundUp(std::max(n, 1), kRowPad), kRowPad);
    s.n = n;
    s.isU8 = u8;
+   s.bfValid = false;
    if (u8)
    {
       if (!s.dRawU8) HIPCHK(c, hipMalloc((void **)&s.dRawU8, (size_t)std::max(s.nMax, 1) * kDim));
@@ -189,7 +207,28 @@ int ensureF32(v3dm_ctx *c, Side &s)
    return V3DM_OK;
 }
 
-int uploadPair(v3dm_ctx *c)
+// BF16 pool and largest squared row norm of a side (float path), from its
+// FP32 form.  Synchronous: the norm feeds the candidate bound on the host.
+int ensureBF16(v3dm_ctx *c, Side &s)
+{
+   if (s.bfValid) return V3DM_OK;
+   int rc = ensureF32(c, s);
+   if (rc) return rc;
+   const int nPad = std::max(roundUp(std::max(s.nMax, 1), kRowPad), kRowPad);
+   if (!s.dBF) HIPCHK(c, hipMalloc((void **)&s.dBF, sizeof(uint16_t) * (size_t)nPad * kDim));
+   const int nPadN = std::max(roundUp(std::max(s.n, 1), kRowPad), kRowPad);
+   HIPCHK(c, launch_prep_bf16(s.dF32, s.n, nPadN, s.dBF, c->dNorm, c->stream));
+   uint32_t bits = 0;
+   HIPCHK(c, hipMemcpyAsync(c->hFlag, c->dNorm, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+   rc = syncStream(c);
+   if (rc) return rc;
+   memcpy(&bits, c->hFlag, sizeof(bits));
+   memcpy(&s.norm2Max, &bits, sizeof(bits));
+   s.bfValid = true;
+   return V3DM_OK;
+}
+
+int uploadPair(v3dm_ctx *c, int variant)
 {
    PairDesc pd{};
    pd.offL = 0;
@@ -198,8 +237,8 @@ int uploadPair(v3dm_ctx *c)
    pd.slotR = 0;
    pd.nL = c->L.n;
    pd.nR = c->R.n;
-   pd.tilesM = (roundUp(pd.nL, kRowPad) + scan_variant_tile_m(0) - 1) / scan_variant_tile_m(0);
-   pd.tilesN = (roundUp(pd.nR, kRowPad) + scan_variant_tile_n(0) - 1) / scan_variant_tile_n(0);
+   pd.tilesM = (roundUp(pd.nL, kRowPad) + scan_variant_tile_m(variant) - 1) / scan_variant_tile_m(variant);
+   pd.tilesN = (roundUp(pd.nR, kRowPad) + scan_variant_tile_n(variant) - 1) / scan_variant_tile_n(variant);
    pd.tileBase = 0;
    pd.finBase = 0;
    if (c->pairValid && memcmp(&pd, &c->hPairUploaded, sizeof(pd)) == 0) return V3DM_OK;
@@ -229,7 +268,7 @@ int chooseKernel(v3dm_ctx *c, int *kernel)
 {
    const bool q = c->L.quantized && c->R.quantized;
    int k = c->kernelSel;
-   if (k == V3DM_KERNEL_AUTO) k = q ? V3DM_KERNEL_I8 : V3DM_KERNEL_PLAIN;
+   if (k == V3DM_KERNEL_AUTO) k = q ? V3DM_KERNEL_I8 : V3DM_KERNEL_BF16;
    if (k == V3DM_KERNEL_I8 && !q)
       return fail(c, V3DM_ERR_UNSUPPORTED,
                   "int8 kernel requested but the descriptors are not all of the form u/512, u in [0,255]");
@@ -240,13 +279,10 @@ int chooseKernel(v3dm_ctx *c, int *kernel)
 // Enqueue the i8 job for the current pair (masks already uploaded).
 int enqueueI8(v3dm_ctx *c, float minScore, float minRatio)
 {
-   int rc = uploadPair(c);
+   int rc = uploadPair(c, 0);
    if (rc) return rc;
-   if (c->hostEpoch >= kEpochLimit)
-   {
-      rc = clearKeys(c);
-      if (rc) return rc;
-   }
+   rc = useKeyFormat(c, 0);
+   if (rc) return rc;
    const PairDesc &pd = c->hPairUploaded;
    ScanArgs a{};
    a.pair0 = pd;
@@ -297,6 +333,76 @@ int enqueueI8(v3dm_ctx *c, float minScore, float minRatio)
    f.minRatio = minRatio;
    f.matchOfLeft = c->dMatch;
    HIPCHK(c, launch_i8_finalize(f, c->stream));
+   if ((rc = profMark(c, 2))) return rc;
+   c->hostEpoch += 1;
+   return V3DM_OK;
+}
+
+// Float path: BF16 candidate scan, FP32 rescoring, finalize (masks already uploaded).
+int enqueueBF16(v3dm_ctx *c, float minScore, float minRatio)
+{
+   int rc = ensureBF16(c, c->L);
+   if (rc) return rc;
+   rc = ensureBF16(c, c->R);
+   if (rc) return rc;
+   const int v = c->bfVariant;
+   rc = uploadPair(c, v);
+   if (rc) return rc;
+   rc = useKeyFormat(c, 1);
+   if (rc) return rc;
+   const PairDesc &pd = c->hPairUploaded;
+   ScanArgsF a{};
+   a.pair0 = pd;
+   a.L = c->L.dBF;
+   a.R = c->R.dBF;
+   a.Lf = c->L.dF32;
+   a.Rf = c->R.dF32;
+   a.pairs = c->dPair;
+   a.nPairs = 1;
+   a.nTiles = (pd.nL > 0 && pd.nR > 0) ? pd.tilesM * pd.tilesN : 0;
+   a.thr = survivorThresholdF(minScore, minRatio);
+   a.lim = bf16CandidateLimit(a.thr, c->L.norm2Max, c->R.norm2Max);
+   c->lastThr = a.thr;
+   c->lastLim = a.lim;
+   a.top1L = c->L.dTop1;
+   a.top2L = c->L.dTop2;
+   a.top1R = c->R.dTop1;
+   a.top2R = c->R.dTop2;
+   a.epoch = c->dEpoch;
+   a.masks = c->dMasks;
+   a.nMasks = (int)c->masks.size();
+   a.surv = c->dSurv;
+   a.survCount = c->dSurvCount;
+   a.survCap = c->survCap;
+   a.waves = scan_variant_waves(v);
+   a.tileM = scan_variant_tile_m(v);
+   a.tileN = scan_variant_tile_n(v);
+   a.persistent = 0;
+   a.tileCounter = c->dEpoch + 2;
+   hipEvent_t e0, e1;
+   if ((rc = profKernelEvents(c, &e0, &e1))) return rc;
+   if (a.nTiles <= 0 && e0)
+   {
+      HIPCHK(c, hipEventRecord(e0, c->stream));
+      HIPCHK(c, hipEventRecord(e1, c->stream));
+   }
+   HIPCHK(c, launch_bf16_scan(v, a.nTiles, a, c->stream, e0, e1));
+   HIPCHK(c, launch_f32_rescore(a, c->stream));
+   FinalizeArgs f{};
+   f.pairs = c->dPair;
+   f.nPairs = 1;
+   f.nBlocks = std::max(1, (pd.nL + 255) / 256);
+   f.top1L = c->L.dTop1;
+   f.top2L = c->L.dTop2;
+   f.top1R = c->R.dTop1;
+   f.top2R = c->R.dTop2;
+   f.epoch = c->dEpoch;
+   f.done = c->dEpoch + 1;
+   f.tileCounter = c->dEpoch + 2;
+   f.minScore = minScore;
+   f.minRatio = minRatio;
+   f.matchOfLeft = c->dMatch;
+   HIPCHK(c, launch_f32_finalize(f, c->stream));
    if ((rc = profMark(c, 2))) return rc;
    c->hostEpoch += 1;
    return V3DM_OK;
@@ -371,7 +477,9 @@ int runMatch(v3dm_ctx *c, float minScore, float minRatio, bool fromFind)
       c->masks.clear();
       c->sValid = false;
    }
-   rc = kernel == V3DM_KERNEL_I8 ? enqueueI8(c, minScore, minRatio) : enqueuePlain(c, minScore, minRatio);
+   if (kernel == V3DM_KERNEL_I8) rc = enqueueI8(c, minScore, minRatio);
+   else if (kernel == V3DM_KERNEL_BF16) rc = enqueueBF16(c, minScore, minRatio);
+   else rc = enqueuePlain(c, minScore, minRatio);
    if (rc) return rc;
    c->lastKernel = kernel;
    c->lastNL = c->L.n;
@@ -546,6 +654,7 @@ int v3dm_allocate(v3dm_ctx *c, int nMaxLeft, int nMaxRight, int useGuided)
       return rc;
    }
    const int nPadL = std::max(roundUp(nMaxLeft, kRowPad), kRowPad);
+   HIPCHK(c, hipMalloc((void **)&c->dNorm, sizeof(uint32_t)));
    HIPCHK(c, hipMalloc((void **)&c->dPair, sizeof(PairDesc)));
    HIPCHK(c, hipMalloc((void **)&c->dEpoch, 4 * sizeof(uint32_t))); // epoch, done, tile counter
    HIPCHK(c, hipMalloc((void **)&c->dMatch, sizeof(int32_t) * nPadL));
@@ -554,10 +663,15 @@ int v3dm_allocate(v3dm_ctx *c, int nMaxLeft, int nMaxRight, int useGuided)
    HIPCHK(c, hipMalloc((void **)&c->dColMax, sizeof(int32_t) * nPadL));
    HIPCHK(c, hipMalloc((void **)&c->dRowMax, sizeof(int32_t) * std::max(roundUp(nMaxRight, kRowPad), kRowPad)));
    {
-      const int64_t tiles =
-         (int64_t)((roundUp(std::max(nMaxLeft, 1), kRowPad) + scan_variant_tile_m(0) - 1) / scan_variant_tile_m(0)) *
-         ((roundUp(std::max(nMaxRight, 1), kRowPad) + scan_variant_tile_n(0) - 1) / scan_variant_tile_n(0));
-      const int64_t regions = tiles * scan_variant_waves(0);
+      // survivor regions for the largest count any scan variant needs
+      int64_t regions = 0;
+      for (int v = 0; v < scan_variant_count(); ++v)
+      {
+         const int64_t tiles =
+            (int64_t)((roundUp(std::max(nMaxLeft, 1), kRowPad) + scan_variant_tile_m(v) - 1) / scan_variant_tile_m(v)) *
+            ((roundUp(std::max(nMaxRight, 1), kRowPad) + scan_variant_tile_n(v) - 1) / scan_variant_tile_n(v));
+         regions = std::max(regions, tiles * scan_variant_waves(v));
+      }
       HIPCHK(c, hipMalloc((void **)&c->dSurv, sizeof(uint64_t) * (size_t)regions * c->survCap));
       HIPCHK(c, hipMalloc((void **)&c->dSurvCount, sizeof(int32_t) * (size_t)regions));
    }
@@ -629,7 +743,7 @@ int v3dm_run_guided_homography_matching(v3dm_ctx *c, const double H[9], float th
 int v3dm_set_kernel(v3dm_ctx *c, int kernel)
 {
    if (!c) return V3DM_ERR_ARG;
-   if (kernel < V3DM_KERNEL_AUTO || kernel > V3DM_KERNEL_I8)
+   if (kernel < V3DM_KERNEL_AUTO || kernel > V3DM_KERNEL_BF16)
       return fail(c, V3DM_ERR_ARG, "set_kernel: unknown kernel %d", kernel);
    c->kernelSel = kernel;
    return V3DM_OK;
@@ -726,6 +840,43 @@ int v3dm_mfma_layout_probe(int device, int32_t *out)
    if (e == hipSuccess) e = hipMemcpy(out, d, sizeof(int32_t) * 1024, hipMemcpyDeviceToHost);
    if (d) (void)hipFree(d);
    if (e != hipSuccess) return fail(nullptr, V3DM_ERR_DEVICE, "mfma_layout_probe: %s", hipGetErrorString(e));
+   return V3DM_OK;
+}
+
+int v3dm_fetch_left_top2(v3dm_ctx *c, int n, float *best, float *second, int32_t *argBest)
+{
+   if (!c) return V3DM_ERR_ARG;
+   if (!c->allocated) return fail(c, V3DM_ERR_STATE, "fetch_left_top2: allocate() has not been called");
+   if (c->lastKernel != V3DM_KERNEL_I8 && c->lastKernel != V3DM_KERNEL_BF16)
+      return fail(c, V3DM_ERR_STATE, "fetch_left_top2: the last job did not run a top-2 kernel");
+   if (n < 0 || n > c->lastNL) return fail(c, V3DM_ERR_ARG, "fetch_left_top2: n = %d out of range", n);
+   HIPCHK(c, hipSetDevice(c->device));
+   int rc = syncStream(c);
+   if (rc) return rc;
+   c->pendingAsync = false;
+   std::vector<uint64_t> t1(std::max(n, 1)), t2(std::max(n, 1));
+   if (n > 0)
+   {
+      HIPCHK(c, hipMemcpy(t1.data(), c->L.dTop1, sizeof(uint64_t) * n, hipMemcpyDeviceToHost));
+      HIPCHK(c, hipMemcpy(t2.data(), c->L.dTop2, sizeof(uint64_t) * n, hipMemcpyDeviceToHost));
+   }
+   const uint32_t ep = c->hostEpoch - 1; // epoch of the last finished job
+   const bool fl = c->keyFormat == 1;
+   for (int k = 0; k < n; ++k)
+   {
+      float b = -1e30f, s2 = -1e30f;
+      int32_t arg = -1;
+      if ((fl ? keyf_epoch(t1[k]) : key_epoch(t1[k])) == ep)
+      {
+         b = fl ? keyf_score(t1[k]) : (float)key_score(t1[k]) * (1.0f / 262144.0f);
+         arg = (int32_t)key_index(t1[k]);
+      }
+      if ((fl ? keyf_epoch(t2[k]) : key_epoch(t2[k])) == ep)
+         s2 = fl ? keyf_score(t2[k]) : (float)key_score(t2[k]) * (1.0f / 262144.0f);
+      if (best) best[k] = b;
+      if (second) second[k] = s2;
+      if (argBest) argBest[k] = arg;
+   }
    return V3DM_OK;
 }
 

@@ -22,6 +22,12 @@ int scan_variant_tile_m(int v);
 int scan_variant_tile_n(int v);
 int scan_variant_resident(int v, int device);
 hipError_t launch_i8_finalize(const FinalizeArgs &a, hipStream_t st);
+hipError_t launch_bf16_scan(int variant, int grid, const ScanArgsF &a, hipStream_t st, hipEvent_t start = nullptr,
+                            hipEvent_t stop = nullptr);
+int bf16_scan_resident(int variant, int device);
+hipError_t launch_f32_rescore(const ScanArgsF &a, hipStream_t st);
+hipError_t launch_f32_finalize(const FinalizeArgs &a, hipStream_t st);
+hipError_t launch_prep_bf16(const float *src, int n, int nPad, uint16_t *dst, uint32_t *norm2Max, hipStream_t st);
 hipError_t launch_i8_insert(const ScanArgs &a, hipStream_t st);
 hipError_t launch_clear_keys(uint64_t *keys, int64_t n, uint32_t *epoch, uint32_t *done, hipStream_t st);
 hipError_t launch_mfma_probe(int32_t *out, hipStream_t st);
@@ -68,6 +74,37 @@ inline int32_t survivorThreshold(float minScore, float minRatio)
    return (int32_t)x;
 }
 
+// Float form of the survivor threshold (same proof): the largest float <= t,
+// or -inf when nothing may be filtered.
+inline float survivorThresholdF(float minScore, float minRatio)
+{
+   double t;
+   if (minRatio < 0)
+      t = (double)minScore;
+   else if (minRatio > 0)
+      t = std::min((double)minScore, 1.0 - (1.0 - (double)minScore) / ((double)minRatio * (1.0 - 1e-5)));
+   else
+      return -INFINITY;
+   if (!(t == t)) return -INFINITY;
+   float f = (float)t;
+   if ((double)f > t) f = std::nextafter(f, -INFINITY);
+   return f;
+}
+
+// BF16 candidate limit: thr minus the BF16 error bound (2^-7 + 2^-14) |x| |y|
+// for the largest row norms, rounded down.
+inline float bf16CandidateLimit(float thr, float norm2MaxL, float norm2MaxR)
+{
+   if (thr == -INFINITY) return -INFINITY;
+   const double delta = (std::ldexp(1.0, -7) + std::ldexp(1.0, -14)) *
+                        std::sqrt((double)norm2MaxL * (double)norm2MaxR) * (1.0 + 1e-4);
+   const double l = (double)thr - delta;
+   if (!(l == l) || l < -3.0e38) return -INFINITY;
+   float f = (float)l;
+   if ((double)f > l) f = std::nextafter(f, -INFINITY);
+   return f;
+}
+
 struct Side
 {
    int n = 0;
@@ -81,6 +118,9 @@ struct Side
    int32_t *dSum = nullptr;
    uint64_t *dTop1 = nullptr, *dTop2 = nullptr;
    float *dPos = nullptr; // kMaxMasks position sets (guided)
+   uint16_t *dBF = nullptr; // BF16 pool (float path), rows padded to kRowPad
+   bool bfValid = false;
+   float norm2Max = 0.0f;   // largest squared row norm (BF16 error bound)
 };
 
 } // namespace v3dm_host
@@ -134,6 +174,12 @@ struct v3dm_ctx
    uint64_t *dSurv = nullptr;
    int32_t *dSurvCount = nullptr;
    int survCap = 64;
+
+   // float path
+   int bfVariant = 2;         // scan variant of the BF16 kernel
+   uint32_t *dNorm = nullptr; // largest squared row norm, written by the BF16 prep kernel
+   int keyFormat = 0;         // 0: integer-score keys in the top-2 arrays, 1: float-score keys
+   float lastThr = 0.0f, lastLim = 0.0f;
 
    // kernel timing: three events per job (before scan, after scan, after select)
    bool profile = false;

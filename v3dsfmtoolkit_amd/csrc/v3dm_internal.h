@@ -46,6 +46,29 @@ __host__ __device__ inline uint32_t key_epoch(uint64_t k) { return (uint32_t)(k 
 __host__ __device__ inline uint32_t key_score(uint64_t k) { return (uint32_t)((k >> 21) & 0x7FFFFFu); }
 __host__ __device__ inline uint32_t key_index(uint64_t k) { return (uint32_t)(kMaxLocalIndex - (uint32_t)(k & 0x1FFFFFu)); }
 
+// Float-score key (BF16 candidate path): [63:53] epoch | [52:21] order-preserving
+// image of the FP32 score | [20:0] (kMaxLocalIndex - index).  -0 is folded into
+// +0 first, so equal scores compare equal as in the reference's `>`.
+constexpr uint32_t kEpochLimitF = (1u << 11) - 2;
+__host__ __device__ inline uint32_t float_order(float s)
+{
+   union { float f; uint32_t u; } v;
+   v.f = s == 0.0f ? 0.0f : s;
+   return (v.u & 0x80000000u) ? ~v.u : (v.u | 0x80000000u);
+}
+__host__ __device__ inline float float_unorder(uint32_t o)
+{
+   union { float f; uint32_t u; } v;
+   v.u = (o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o;
+   return v.f;
+}
+__host__ __device__ inline uint64_t pack_keyf(uint32_t epoch, float score, uint32_t index)
+{
+   return ((uint64_t)epoch << 53) | ((uint64_t)float_order(score) << 21) | (uint64_t)(kMaxLocalIndex - index);
+}
+__host__ __device__ inline uint32_t keyf_epoch(uint64_t k) { return (uint32_t)(k >> 53); }
+__host__ __device__ inline float keyf_score(uint64_t k) { return float_unorder((uint32_t)(k >> 21)); }
+
 struct ScanArgs
 {
    PairDesc pair0;              // the pair itself when nPairs == 1 (no lookup)
@@ -68,6 +91,29 @@ struct ScanArgs
    int waves, tileM, tileN;     // tile geometry of the launched scan variant
    int persistent;              // 1: resident workgroups pull tiles from *tileCounter
    uint32_t *tileCounter;       // reset to 0 by the finalize kernel
+};
+
+// Float descriptors: BF16 MFMA candidate scan + exact FP32 rescoring.
+struct ScanArgsF
+{
+   PairDesc pair0;           // the pair itself when nPairs == 1
+   const uint16_t *L, *R;    // BF16 pools (round-to-nearest-even), rows padded to kRowPad with zeros
+   const float *Lf, *Rf;     // FP32 pools, the descriptors as given (rescoring)
+   const PairDesc *pairs;
+   int nPairs;
+   int nTiles;
+   float lim;                // candidate iff BF16 score > lim (= thr - proven BF16 error bound)
+   float thr;                // a rescored FP32 score enters the top-2 iff > thr
+   uint64_t *top1L, *top2L, *top1R, *top2R;
+   const uint32_t *epoch;
+   const Mask *masks;
+   int nMasks;
+   uint64_t *surv;
+   int32_t *survCount;
+   int survCap;
+   int waves, tileM, tileN;
+   int persistent;
+   uint32_t *tileCounter;
 };
 
 struct FinalizeArgs

@@ -116,6 +116,17 @@ class SIFTMatching:
         check(self._lib.v3dm_profile_read(self._ctx, ctypes.byref(a), ctypes.byref(b), ctypes.byref(n)), self._ctx)
         return a.value, b.value, n.value
 
+    def leftTop2(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        """(best, second, argBest) per left feature of the last top-2 job,
+        over the candidates above the survivor threshold (-1e30 / -1 where none)."""
+        n_left = self.counts()[0]
+        best = np.empty(n_left, np.float32)
+        second = np.empty(n_left, np.float32)
+        arg = np.empty(n_left, np.int32)
+        check(self._lib.v3dm_fetch_left_top2(self._ctx, n_left, ptr(best, ctypes.c_float), ptr(second, ctypes.c_float),
+                                             ptr(arg, ctypes.c_int32)), self._ctx)
+        return best, second, arg
+
     # -- many pairs per job (v3dm_batch_*) ---------------------------------------
     def setImages(self, images) -> list[int]:
         """Uploads uint8 descriptor sets (list of (n_i, 128) arrays); returns
