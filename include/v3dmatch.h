@@ -220,6 +220,46 @@ int v3dm_device_count(int *count);
  * register v of lane l holds row 8*(v/4) + 4*(l/32) + v%4, column l%32. */
 int v3dm_mfma_layout_probe(int device, int32_t *out);
 
+/* ---- learned proposal path (no counterpart in the reference) -------------
+ *
+ * An embedding network (128 -> 256 GELU -> 64, L2-normalised, BF16 weights,
+ * input u / 64) trained on the GPU on matched descriptor pairs with a
+ * symmetric InfoNCE loss, and a matcher that scores only each left's top-16
+ * rights in the embedding space (exact int8 scores), rescans every left or
+ * right query whose best - second margin is under `margin` exhaustively, and
+ * applies the reference rule (v3d_cudamatching.cu:117-194,
+ * v3d_cudamatching.cpp:121-126) to the exact scores of the scored pairs.
+ * Parameter layout: W1 (256 x 128, row-major) | b1 (256) | W2 (64 x 256) | b2 (64).
+ * A network object is not thread-safe. */
+typedef struct v3dm_mlp v3dm_mlp;
+int v3dm_mlp_param_count(void);
+int v3dm_mlp_create(int device, uint64_t seed, v3dm_mlp **out);
+void v3dm_mlp_destroy(v3dm_mlp *net);
+const char *v3dm_mlp_last_error(const v3dm_mlp *net);
+int v3dm_mlp_get_params(v3dm_mlp *net, float *params);
+/* Sets the FP32 master parameters and resets the optimiser state. */
+int v3dm_mlp_set_params(v3dm_mlp *net, const float *params);
+/* Loss and parameter gradient of one batch of matched pairs (left[i], right[i]),
+ * batch x 128 uint8 each, at the given temperature; no update. */
+int v3dm_mlp_loss_grad(v3dm_mlp *net, int batch, const uint8_t *left, const uint8_t *right, float temperature,
+                       float *loss, float *grad);
+/* Uploads the training set: n matched pairs, n x 128 uint8 per side. */
+int v3dm_mlp_set_training_pairs(v3dm_mlp *net, int64_t n, const uint8_t *left, const uint8_t *right);
+/* `steps` Adam steps (betas 0.9 / 0.999), each on `batch` pairs drawn with
+ * replacement (seeded); losses[steps] receives the batch losses (may be NULL). */
+int v3dm_mlp_train(v3dm_mlp *net, int steps, int batch, float lr, float temperature, uint64_t seed, float *losses);
+/* Embeddings (n x 64 floats) of n uint8 descriptors. */
+int v3dm_mlp_embed(v3dm_mlp *net, int n, const uint8_t *desc, float *out);
+/* Learned matching of the context's current pair (both sides uploaded as
+ * uint8).  Output as v3dm_find_putative_matches; stats (may be NULL) receives
+ * [nLeft, nRight, candidate pairs, flagged lefts, flagged rights, pairs
+ * scored by the fallback scans]. */
+int v3dm_learned_find(v3dm_ctx *ctx, v3dm_mlp *net, float margin, float minScore, float minRatio, int32_t *pairs,
+                      int capacity, int *count, int64_t *stats);
+/* Diagnostic: the last learned job's candidate table, n x 16 right indices
+ * (-1 for none), n <= the left count of that job. */
+int v3dm_learned_candidates(v3dm_ctx *ctx, int n, int32_t *cand);
+
 /* Diagnostic, no counterpart in the reference: per left feature k < n of the
  * last completed match job, the best and second-best score among the
  * candidates that reached the exact top-2 (scores above the survivor

@@ -5,7 +5,7 @@ behind a C ABI, include/v3dmatch.h); this package is its Python binding.
 """
 from ._lib import (KERNEL_AUTO, KERNEL_BF16, KERNEL_I8, KERNEL_PLAIN, V3DMError, copy_probe, device_count, lib,
                    mfma_layout_probe, scan_variants, synth_pair, synth_seed)
-from .matching import SIFTMatching
+from .matching import EmbeddingNet, SIFTMatching
 
-__all__ = ["SIFTMatching", "V3DMError", "KERNEL_AUTO", "KERNEL_PLAIN", "KERNEL_I8", "KERNEL_BF16", "device_count", "lib",
+__all__ = ["SIFTMatching", "EmbeddingNet", "V3DMError", "KERNEL_AUTO", "KERNEL_PLAIN", "KERNEL_I8", "KERNEL_BF16", "device_count", "lib",
            "mfma_layout_probe", "copy_probe", "scan_variants", "synth_pair", "synth_seed"]

@@ -83,6 +83,21 @@ _SIGNATURES = {
     "v3dm_mfma_layout_probe": (ctypes.c_int, [ctypes.c_int, _i32_p]),
     "v3dm_copy_probe": (ctypes.c_int, [ctypes.c_int, ctypes.c_int64, ctypes.c_int]),
     "v3dm_fetch_left_top2": (ctypes.c_int, [_ctx_p, ctypes.c_int, _f32_p, _f32_p, _i32_p]),
+    "v3dm_mlp_param_count": (ctypes.c_int, []),
+    "v3dm_mlp_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)]),
+    "v3dm_mlp_destroy": (None, [ctypes.c_void_p]),
+    "v3dm_mlp_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
+    "v3dm_mlp_get_params": (ctypes.c_int, [ctypes.c_void_p, _f32_p]),
+    "v3dm_mlp_set_params": (ctypes.c_int, [ctypes.c_void_p, _f32_p]),
+    "v3dm_mlp_loss_grad": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _u8_p, _u8_p, ctypes.c_float, _f32_p,
+                                          _f32_p]),
+    "v3dm_mlp_set_training_pairs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, _u8_p, _u8_p]),
+    "v3dm_mlp_train": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float,
+                                      ctypes.c_uint64, _f32_p]),
+    "v3dm_mlp_embed": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _u8_p, _f32_p]),
+    "v3dm_learned_find": (ctypes.c_int, [_ctx_p, ctypes.c_void_p, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                         _i32_p, ctypes.c_int, _c_int_p, _i64_p]),
+    "v3dm_learned_candidates": (ctypes.c_int, [_ctx_p, ctypes.c_int, _i32_p]),
 }
 
 _lib = None

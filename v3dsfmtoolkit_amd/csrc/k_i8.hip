@@ -57,6 +57,9 @@ __device__ __forceinline__ void survivor(const ScanArgs &a, const PairDesc &pd, 
                                          int32_t score)
 {
    if (il >= pd.nL || jr >= pd.nR) return;
+   // rows of a gathered pool stand for the features mapL / mapR name
+   if (a.mapL) il = a.mapL[il];
+   if (a.mapR) jr = a.mapR[jr];
    for (int m = 0; m < a.nMasks; ++m)
    {
       const Mask &mk = a.masks[m];

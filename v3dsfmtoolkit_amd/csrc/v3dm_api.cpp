@@ -67,6 +67,7 @@ int profKernelEvents(v3dm_ctx *c, hipEvent_t *start, hipEvent_t *stop)
 
 void freeAll(v3dm_ctx *c)
 {
+   freeLearned(c);
    freeSide(c->L);
    freeSide(c->R);
    dfree(c->dPair);

@@ -212,8 +212,8 @@ void fillArgs(v3dm_ctx *c, BatchState *b, float minScore, float minRatio, ScanAr
 
 int enqueueKernels(v3dm_ctx *c, BatchState *b, float minScore, float minRatio, bool marks)
 {
-   ScanArgs a;
-   FinalizeArgs f;
+   ScanArgs a{};
+   FinalizeArgs f{};
    int grid = 0;
    fillArgs(c, b, minScore, minRatio, a, f, grid);
    int rc;

@@ -127,6 +127,7 @@ struct Side
 
 namespace v3dm_host {
 struct BatchState;
+struct LearnedState;
 }
 
 using namespace v3dm_host;
@@ -186,7 +187,8 @@ struct v3dm_ctx
    int profJobs = 0;
    std::vector<hipEvent_t> events;
 
-   v3dm_host::BatchState *batch = nullptr; // many-pair jobs (v3dm_batch_*)
+   v3dm_host::BatchState *batch = nullptr;     // many-pair jobs (v3dm_batch_*)
+   v3dm_host::LearnedState *learned = nullptr; // learned proposal path (v3dm_learned_find)
 };
 
 namespace v3dm_host {
@@ -222,5 +224,9 @@ int syncStream(v3dm_ctx *c);
 int profMark(v3dm_ctx *c, int which);
 int profKernelEvents(v3dm_ctx *c, hipEvent_t *start, hipEvent_t *stop);
 void freeBatch(v3dm_ctx *c);
+void freeLearned(v3dm_ctx *c);
+int uploadPair(v3dm_ctx *c, int variant);
+int useKeyFormat(v3dm_ctx *c, int format);
+int fetch(v3dm_ctx *c, int32_t *pairs, int capacity, int *count);
 
 } // namespace v3dm_host

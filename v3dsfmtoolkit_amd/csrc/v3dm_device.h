@@ -64,10 +64,12 @@ __device__ inline bool mask_hit(const Mask &mk, float px, float py, float qx, fl
    return sqrDist > mk.sqrThr;
 }
 
+// Exact top-2 of unique keys; a key inserted again (the same pair scored by
+// two passes) leaves both slots as they are.
 __device__ inline void top2_insert(uint64_t *t1, uint64_t *t2, uint64_t key)
 {
    unsigned long long old = atomicMax((unsigned long long *)t1, (unsigned long long)key);
-   atomicMax((unsigned long long *)t2, (unsigned long long)(old < key ? old : key));
+   if (old != key) atomicMax((unsigned long long *)t2, (unsigned long long)(old < key ? old : key));
 }
 
 __device__ inline v4i ld16(const int8_t *p)

@@ -91,6 +91,7 @@ struct ScanArgs
    int waves, tileM, tileN;     // tile geometry of the launched scan variant
    int persistent;              // 1: resident workgroups pull tiles from *tileCounter
    uint32_t *tileCounter;       // reset to 0 by the finalize kernel
+   const int32_t *mapL, *mapR;  // optional: feature index of each row of a gathered pool
 };
 
 // Float descriptors: BF16 MFMA candidate scan + exact FP32 rescoring.

@@ -127,6 +127,25 @@ class SIFTMatching:
                                              ptr(arg, ctypes.c_int32)), self._ctx)
         return best, second, arg
 
+    # -- learned proposal path ----------------------------------------------------
+    def learnedFind(self, net: "EmbeddingNet", margin: float = 0.05, minScore: float = 0.9,
+                    minRatio: float = 0.8) -> tuple[list[tuple[int, int]], dict]:
+        """Matches the current pair through the learned proposal path (both
+        sides uploaded as uint8).  Returns (pairs, stats)."""
+        pairs, count = self._buffer()
+        st = np.zeros(6, np.int64)
+        check(self._lib.v3dm_learned_find(self._ctx, net._h, margin, minScore, minRatio, ptr(pairs, ctypes.c_int32),
+                                          self._cap, ctypes.byref(count), ptr(st, ctypes.c_int64)), self._ctx)
+        keys = ["n_left", "n_right", "candidates", "flagged_left", "flagged_right", "fallback_pairs"]
+        return _as_list(pairs, count.value), {k: int(x) for k, x in zip(keys, st)}
+
+    def learnedCandidates(self) -> np.ndarray:
+        """(n_left, 16) candidate right indices of the last learned job."""
+        n = self.counts()[0]
+        out = np.empty((n, 16), np.int32)
+        check(self._lib.v3dm_learned_candidates(self._ctx, n, ptr(out, ctypes.c_int32)), self._ctx)
+        return out
+
     # -- many pairs per job (v3dm_batch_*) ---------------------------------------
     def setImages(self, images) -> list[int]:
         """Uploads uint8 descriptor sets (list of (n_i, 128) arrays); returns
@@ -235,3 +254,76 @@ class SIFTMatching:
 
 def _as_list(pairs: np.ndarray, n: int) -> list[tuple[int, int]]:
     return [(int(a), int(b)) for a, b in pairs[:n]]
+
+
+class EmbeddingNet:
+    """The learned proposal network (v3dm_mlp_*): 128 -> 256 (GELU) -> 64,
+    L2-normalised, BF16 weights; trained and run by the library's HIP kernels."""
+
+    PARAMS = None
+
+    def __init__(self, device: int = 0, seed: int = 1):
+        self._lib = _lib.lib()
+        h = ctypes.c_void_p()
+        rc = self._lib.v3dm_mlp_create(device, seed, ctypes.byref(h))
+        if rc != _lib.V3DM_OK:
+            raise _lib.V3DMError(rc, self._lib.v3dm_mlp_last_error(None).decode(errors="replace"))
+        self._h = h
+        self.n_params = self._lib.v3dm_mlp_param_count()
+
+    def _check(self, rc):
+        if rc != _lib.V3DM_OK:
+            raise _lib.V3DMError(rc, self._lib.v3dm_mlp_last_error(self._h).decode(errors="replace"))
+
+    def params(self) -> np.ndarray:
+        out = np.empty(self.n_params, np.float32)
+        self._check(self._lib.v3dm_mlp_get_params(self._h, ptr(out, ctypes.c_float)))
+        return out
+
+    def set_params(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.n_params
+        self._check(self._lib.v3dm_mlp_set_params(self._h, ptr(p, ctypes.c_float)))
+
+    def loss_grad(self, left: np.ndarray, right: np.ndarray, temperature: float) -> tuple[float, np.ndarray]:
+        left = np.ascontiguousarray(left, np.uint8)
+        right = np.ascontiguousarray(right, np.uint8)
+        assert left.shape == right.shape and left.shape[1] == 128
+        loss = np.zeros(1, np.float32)
+        grad = np.empty(self.n_params, np.float32)
+        self._check(self._lib.v3dm_mlp_loss_grad(self._h, left.shape[0], ptr(left, ctypes.c_uint8),
+                                                 ptr(right, ctypes.c_uint8), temperature, ptr(loss, ctypes.c_float),
+                                                 ptr(grad, ctypes.c_float)))
+        return float(loss[0]), grad
+
+    def set_training_pairs(self, left: np.ndarray, right: np.ndarray) -> None:
+        left = np.ascontiguousarray(left, np.uint8)
+        right = np.ascontiguousarray(right, np.uint8)
+        assert left.shape == right.shape and left.shape[1] == 128
+        self._check(self._lib.v3dm_mlp_set_training_pairs(self._h, left.shape[0], ptr(left, ctypes.c_uint8),
+                                                          ptr(right, ctypes.c_uint8)))
+
+    def train(self, steps: int, batch: int = 1024, lr: float = 1e-3, temperature: float = 0.05,
+              seed: int = 7) -> np.ndarray:
+        losses = np.zeros(max(steps, 1), np.float32)
+        self._check(self._lib.v3dm_mlp_train(self._h, steps, batch, lr, temperature, seed,
+                                             ptr(losses, ctypes.c_float)))
+        return losses[:steps]
+
+    def embed(self, desc: np.ndarray) -> np.ndarray:
+        desc = np.ascontiguousarray(desc, np.uint8)
+        out = np.empty((desc.shape[0], 64), np.float32)
+        self._check(self._lib.v3dm_mlp_embed(self._h, desc.shape[0], ptr(desc, ctypes.c_uint8),
+                                             ptr(out, ctypes.c_float)))
+        return out
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.v3dm_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
