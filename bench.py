@@ -16,6 +16,8 @@ import statistics
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -82,6 +84,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run a plain device copy of this size (scales memory counters)")
+    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5"], default="c2",
+                    help="c2: single pair 8192^2 (default line); c3: 256 uneven pairs in one batch (+ float "
+                         "variant); c4: learned path on 64 pairs of 16384^2; c5: single pair 65536^2")
+    ap.add_argument("--pairs", type=int, default=0, help="number of pairs for c3 / c4 (0: the workload's own)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -91,6 +97,14 @@ def main():
 
     import v3dsfmtoolkit_amd as v
 
+    if args.workload != "c2":
+        out = {"c3": run_c3, "c4": run_c4, "c5": run_c5}[args.workload](args, v)
+        print(json.dumps(out), flush=True)
+        return
+    run_c2(args, v)
+
+
+def run_c2(args, v):
     n = args.n
     min_score, min_ratio = 0.8, 0.8
     d = v.synth_pair(v.synth_seed(1, 0), n, n, 0.4, 0.05)
@@ -190,6 +204,273 @@ def main():
     if args.copy_probe_mib > 0:
         copy_probe(args.copy_probe_mib)
     print(json.dumps(out), flush=True)
+
+
+def _common(metric, value, unit, steps, warmup, wall, dtype, config, higher=True):
+    return {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps, "warmup": warmup,
+            "ms_per_step": wall * 1e3, "higher_is_better": higher, "scaling": "weak", "vs_baseline": None,
+            "dtype": dtype, "data": "synthetic", "config": config}
+
+
+def run_c3(args, v):
+    """256 image pairs, feature counts uniform in [500, 12000], one batched
+    job (persistent tile queue, autotuned tile shape, hipGraph replay); then
+    the float-descriptor variant (BF16 candidates + FP32 rescoring) on a
+    sample of the pairs against the FP64 oracle."""
+    from oracle import oracle as O
+
+    n_pairs = args.pairs or 256
+    rng = np.random.default_rng(2024)
+    counts = rng.integers(500, 12001, size=(n_pairs, 2))
+    images, fimages = [], []
+    progress(f"generating {n_pairs} pairs")
+    for i in range(n_pairs):
+        d = v.synth_pair(v.synth_seed(3, i), int(counts[i, 0]), int(counts[i, 1]), 0.4, 0.05, f32=i < 16)
+        images += [d["L"], d["R"]]
+        if i < 16:
+            fimages.append((d["Lf"], d["Rf"]))
+    m = v.SIFTMatching(0)
+    m.enableRoundingWarning(False)
+    c32 = m.setImages(images)
+    m.setPairs([(2 * i, 2 * i + 1) for i in range(n_pairs)])
+    pairs_scored = int(sum(c32[2 * i] * c32[2 * i + 1] for i in range(n_pairs)))
+    best, tune_ms = m.batchAutotune(0.8, 0.8, reps=2)
+    m.batchSetVariant(best)
+    # scan kernel time (events bound to the scan dispatch, eager jobs)
+    m.batchSetGraph(False)
+    m.profileEnable(True)
+    for _ in range(5):
+        m.batchMatchAsync(0.8, 0.8)
+    m.batchFetch()
+    scan_ms, sel_ms, jobs = m.profileRead()
+    m.profileEnable(False)
+    m.batchSetGraph(True)
+    tabs = m.batchMatch(0.8, 0.8)
+    warm = max(args.warmup // 20, 3)
+    for _ in range(warm):
+        m.batchMatchAsync(0.8, 0.8)
+    m.batchFetch()
+    steps = max(args.steps // 100, 10)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        m.batchMatchAsync(0.8, 0.8)
+    tabs2 = m.batchFetch()
+    wall = (time.perf_counter() - t0) / steps
+    assert all(np.array_equal(a, b) for a, b in zip(tabs, tabs2))
+    # parity and CPU baseline on a sample of pairs
+    sample = list(range(4)) + list(rng.choice(np.arange(4, n_pairs), 4, replace=False))
+    cpu_t, cpu_pairs, agree = 0.0, 0, True
+    threads = len(os.sched_getaffinity(0))
+    for i in sample:
+        L, R = images[2 * i][:c32[2 * i]], images[2 * i + 1][:c32[2 * i + 1]]
+        t = time.perf_counter()
+        _, _, tab, used = O.find_u8_blocked(L, R, 0.8, 0.8, threads=threads)
+        cpu_t += time.perf_counter() - t
+        cpu_pairs += len(L) * len(R)
+        agree &= np.array_equal(tab, tabs[i])
+    if not agree:
+        print("bench.py: batched tables differ from the oracle", file=sys.stderr)
+        sys.exit(1)
+    scan_avg = scan_ms / 1e3 / max(jobs, 1)
+    achieved = pairs_scored * 256 / scan_avg / 1e12
+    w, tm, tn = v.scan_variants()[best]
+    out = _common("descriptor pairs scored/s", pairs_scored / wall, "descriptor pairs/s", steps, warm, wall, "int8",
+                  {"workload": f"{n_pairs} image pairs, feature counts uniform in [500, 12000], 40% noisy copies "
+                               "(sigma 0.05), minScore 0.8, minRatio 0.8, mutual check; one batched job",
+                   "pairs": n_pairs, "descriptor_pairs_per_job": pairs_scored, "variant": best,
+                   "tile": [tm, tn], "waves": w, "graph": True})
+    out["image_pairs_per_s"] = n_pairs / wall
+    out["autotune_ms"] = [float(x) for x in tune_ms]
+    out["roofline"] = {"bound": "mfma", "kernel": "k_i8_scan (batched)", "achieved": achieved,
+                       "peak": INT8_PEAK_TOPS, "unit": "TFLOP/s", "frac": achieved / INT8_PEAK_TOPS, "traffic": None,
+                       "launch_avg_us": scan_avg * 1e6}
+    out["cpu_baseline"] = {"value": cpu_pairs / cpu_t, "unit": "descriptor pairs scored/s", "cores": threads,
+                           "kind": "port", "sample": f"{len(sample)} of the pairs ({cpu_pairs} descriptor pairs), "
+                                                     "oracle blocked rule; their tables equal the GPU's"}
+    m.close()
+    progress("batched int8 done; float variant")
+    # float variant: unquantised unit descriptors, single-pair BF16 path per pair
+    fm = v.SIFTMatching(0)
+    fm.enableRoundingWarning(False)
+    fm.allocate(12000, 12000)
+    fm.setKernel(v.KERNEL_BF16)
+    f_wall, f_pairs, max_err, disagree, matches, exact_f32 = 0.0, 0, 0.0, 0, 0, True
+    for Lf, Rf in fimages:
+        nL, nR = len(Lf) - len(Lf) % 32, len(Rf) - len(Rf) % 32
+        fm.setLeftFeatures(nL, Lf)
+        fm.setRightFeatures(nR, Rf)
+        fm.findAsync(0.8, 0.8)
+        fm.synchronize()
+        t = time.perf_counter()
+        for _ in range(5):
+            fm.findAsync(0.8, 0.8)
+        fm.synchronize()
+        f_wall += (time.perf_counter() - t) / 5
+        f_pairs += nL * nR
+        got = fm.fetchMatches()
+        b, _, arg = fm.leftTop2()
+        have = arg >= 0
+        s64 = np.einsum("ij,ij->i", Lf[:nL][have].astype(np.float64), Rf[:nR][arg[have]].astype(np.float64))
+        max_err = max(max_err, float(np.abs(b[have] - s64).max()) if have.any() else 0.0)
+        l64 = set(O.table_to_pairs(O.find_f64(Lf[:nL], Rf[:nR], 0.8, 0.8)[2]))
+        disagree += len(set(got) ^ l64)
+        matches += len(l64)
+        exact_f32 &= got == O.table_to_pairs(O.find_f32(Lf[:nL], Rf[:nR], 0.8, 0.8)[2])
+    fm.close()
+    out["float_variant"] = {"pairs": len(fimages), "descriptor_pairs_per_s": f_pairs / f_wall,
+                            "ms_per_pair": f_wall / len(fimages) * 1e3,
+                            "max_abs_score_error_vs_fp64": max_err,
+                            "list_disagreement_vs_fp64_rule": disagree, "fp64_rule_matches": matches,
+                            "equals_fp32_oracle": bool(exact_f32),
+                            "method": "BF16 MFMA candidates within the proven bound, FP32 rescoring, single-pair jobs"}
+    return out
+
+
+def progress(*a):
+    print("bench.py:", *a, file=sys.stderr, flush=True)
+
+
+def _train_net(v, pairs, rho, sigma, steps, seed=5):
+    from v3dsfmtoolkit_amd import learned as VL
+    t0 = time.perf_counter()
+    L, R = VL.labeled_pairs(pairs, n=16384, rho=rho, sigma=sigma)
+    t1 = time.perf_counter()
+    progress(f"labelled {len(L)} pairs in {t1 - t0:.1f} s")
+    net = v.EmbeddingNet(0, seed=seed)
+    losses = VL.train(net, L, R, steps=steps, batch=1024, lr=2e-3)
+    t2 = time.perf_counter()
+    progress(f"trained {steps} steps in {t2 - t1:.1f} s, loss {losses[0]:.3f} -> {losses[-1]:.3f}")
+    return net, {"pairs": len(L), "steps": steps, "batch": 1024, "label_s": t1 - t0, "train_s": t2 - t1,
+                 "loss_first": float(losses[0]), "loss_last": float(np.mean(losses[-20:]))}
+
+
+def _time_job(fn, reps):
+    fn()
+    t = time.perf_counter()
+    for _ in range(reps):
+        r = fn()
+    return (time.perf_counter() - t) / reps, r
+
+
+def _pairset(a):
+    return set(map(tuple, np.asarray(a).reshape(-1, 2).tolist()))
+
+
+def run_c4(args, v):
+    """Learned proposal path on 64 pairs of 16384 x 16384 (network trained
+    on 2M exact-labelled pairs), beside the exact int8 matcher (the judge of
+    recall) and the plain path."""
+    n_pairs = args.pairs or 64
+    n = 16384
+    net, train = _train_net(v, 2_000_000, 0.4, 0.05, steps=2000)
+    m = v.SIFTMatching(0)
+    m.enableRoundingWarning(False)
+    m.allocate(n, n)
+    tot = {"learned": 0.0, "i8": 0.0, "plain": 0.0, "plain_gpu": 0.0}
+    hit = n_exact = n_learned = 0
+    flagged = [0, 0]
+    fb_pairs = 0
+    for i in range(n_pairs):
+        d = v.synth_pair(v.synth_seed(4, i), n, n, 0.4, 0.05)
+        if i % 8 == 0:
+            progress(f"pair {i} of {n_pairs}")
+        m.setLeftFeatures(n, d["L"])
+        m.setRightFeatures(n, d["R"])
+        m.setKernel(v.KERNEL_I8)
+        exact = m.findPutativeMatches(0.8, 0.8)
+        dt, _ = _time_job(lambda: (m.findAsync(0.8, 0.8), m.synchronize()), 5)
+        tot["i8"] += dt
+        dt, (got, st) = _time_job(lambda: m.learnedFind(net, 0.05, 0.8, 0.8, 16, asArray=True), 3)
+        tot["learned"] += dt
+        g, e = _pairset(got), set(exact)
+        hit += len(g & e)
+        n_exact += len(e)
+        n_learned += len(g)
+        flagged[0] += st["flagged_left"]
+        flagged[1] += st["flagged_right"]
+        fb_pairs += st["fallback_pairs"]
+        if i < 8:  # the plain path is slow: time it on the first pairs
+            m.setKernel(v.KERNEL_PLAIN)
+            m.profileEnable(True)
+            dt, _ = _time_job(lambda: (m.findAsync(0.8, 0.8), m.synchronize()), 2)
+            sm, sl, jobs = m.profileRead()
+            m.profileEnable(False)
+            tot["plain"] += dt
+            tot["plain_gpu"] += (sm + sl) / 1e3 / max(jobs, 1)
+            if m.fetchMatches() != exact:
+                print("bench.py: plain and int8 lists differ", file=sys.stderr)
+                sys.exit(1)
+    m.close()
+    net.close()
+    plain_pp = tot["plain"] / min(n_pairs, 8)
+    learned_pp = tot["learned"] / n_pairs
+    rec, prec = hit / max(n_exact, 1), hit / max(n_learned, 1)
+    out = _common("effective descriptor pairs/s (learned path)", n * n / learned_pp, "descriptor pairs/s",
+                  n_pairs, 1, learned_pp, "bf16 retrieval + int8 rescoring",
+                  {"workload": f"{n_pairs} pairs of {n}x{n} uint8 descriptors, 40% noisy copies (sigma 0.05), "
+                               "minScore 0.8, minRatio 0.8, mutual; top-16 retrieval, margin 0.05 fallback",
+                   "pairs": n_pairs, "n": n, "top_k": 16, "margin": 0.05})
+    out.update({"recall": rec, "precision": prec, "exact_matches": n_exact,
+                "fallback_fraction_left": flagged[0] / (n_pairs * n),
+                "fallback_fraction_right": flagged[1] / (n_pairs * n),
+                "fallback_pairs_per_pair": fb_pairs / n_pairs,
+                "ms_per_pair": {"learned": learned_pp * 1e3, "exact_int8": tot["i8"] / n_pairs * 1e3,
+                                "plain": plain_pp * 1e3, "plain_gpu_events": tot["plain_gpu"] / min(n_pairs, 8) * 1e3},
+                "speedup_vs_plain": plain_pp / learned_pp, "training": train,
+                "timing": "wall clock per job with inputs resident; the learned job includes its one host "
+                          "round trip (flag counts)"})
+    return out
+
+
+def run_c5(args, v):
+    """One pair of 65536 x 65536 (30% overlap, sigma 0.08): exact int8 beside
+    the learned path with top-32 candidates (network trained on pairs of the
+    same generator settings)."""
+    from oracle import oracle as O
+    n = 65536
+    d = v.synth_pair(v.synth_seed(5, 0), n, n, 0.3, 0.08)
+    m = v.SIFTMatching(0)
+    m.enableRoundingWarning(False)
+    m.allocate(n, n)
+    m.setLeftFeatures(n, d["L"])
+    m.setRightFeatures(n, d["R"])
+    m.setKernel(v.KERNEL_I8)
+    exact = m.findPutativeMatches(0.8, 0.8)
+    progress(f"exact int8 list: {len(exact)} matches")
+    m.profileEnable(True)
+    dt_i8, _ = _time_job(lambda: (m.findAsync(0.8, 0.8), m.synchronize()), 10)
+    scan_ms, sel_ms, jobs = m.profileRead()
+    m.profileEnable(False)
+    scan_avg = scan_ms / 1e3 / max(jobs, 1)
+    net, train = _train_net(v, 1_000_000, 0.3, 0.08, steps=1500)
+    dt_l, (got, st) = _time_job(lambda: m.learnedFind(net, 0.05, 0.8, 0.8, 32, asArray=True), 3)
+    g, e = _pairset(got), set(exact)
+    hit = len(g & e)
+    m.setKernel(v.KERNEL_PLAIN)
+    dt_p, _ = _time_job(lambda: (m.findAsync(0.8, 0.8), m.synchronize()), 1)
+    plain_same = m.fetchMatches() == exact
+    m.close()
+    net.close()
+    # CPU baseline: the oracle's blocked rule on a 2048-row slice of the left set
+    threads = len(os.sched_getaffinity(0))
+    t = time.perf_counter()
+    O.find_u8_blocked(d["L"][:2048], d["R"], 0.8, 0.8, threads=threads)
+    cpu = 2048 * n / (time.perf_counter() - t)
+    achieved = n * n * 256 / scan_avg / 1e12
+    out = _common("descriptor pairs scored/s", n * n / dt_i8, "descriptor pairs/s", 10, 1, dt_i8, "int8",
+                  {"workload": f"single pair {n}x{n} uint8, 30% noisy copies (sigma 0.08), minScore 0.8, "
+                               "minRatio 0.8, mutual", "n_left": n, "n_right": n, "matches": len(exact)})
+    out["roofline"] = {"bound": "mfma", "kernel": "k_i8_scan", "achieved": achieved, "peak": INT8_PEAK_TOPS,
+                       "unit": "TFLOP/s", "frac": achieved / INT8_PEAK_TOPS, "traffic": None,
+                       "launch_avg_us": scan_avg * 1e6}
+    out["learned_top32"] = {"ms_per_pair": dt_l * 1e3, "effective_pairs_per_s": n * n / dt_l,
+                            "recall": hit / max(len(e), 1), "precision": hit / max(len(g), 1),
+                            "fallback_fraction_left": st["flagged_left"] / n,
+                            "fallback_fraction_right": st["flagged_right"] / n, "training": train}
+    out["plain"] = {"ms_per_pair": dt_p * 1e3, "equals_int8": plain_same}
+    out["cpu_baseline"] = {"value": cpu, "unit": "descriptor pairs scored/s", "cores": threads, "kind": "port",
+                           "sample": "2048 left rows against all 65536 rights, oracle blocked rule"}
+    return out
 
 
 if __name__ == "__main__":

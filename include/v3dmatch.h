@@ -251,14 +251,15 @@ int v3dm_mlp_train(v3dm_mlp *net, int steps, int batch, float lr, float temperat
 /* Embeddings (n x 64 floats) of n uint8 descriptors. */
 int v3dm_mlp_embed(v3dm_mlp *net, int n, const uint8_t *desc, float *out);
 /* Learned matching of the context's current pair (both sides uploaded as
- * uint8).  Output as v3dm_find_putative_matches; stats (may be NULL) receives
- * [nLeft, nRight, candidate pairs, flagged lefts, flagged rights, pairs
- * scored by the fallback scans]. */
-int v3dm_learned_find(v3dm_ctx *ctx, v3dm_mlp *net, float margin, float minScore, float minRatio, int32_t *pairs,
-                      int capacity, int *count, int64_t *stats);
-/* Diagnostic: the last learned job's candidate table, n x 16 right indices
- * (-1 for none), n <= the left count of that job. */
-int v3dm_learned_candidates(v3dm_ctx *ctx, int n, int32_t *cand);
+ * uint8), topK = 16 or 32 candidates per left.  Output as
+ * v3dm_find_putative_matches; stats (may be NULL) receives [nLeft, nRight,
+ * candidate pairs, flagged lefts, flagged rights, pairs scored by the
+ * fallback scans]. */
+int v3dm_learned_find(v3dm_ctx *ctx, v3dm_mlp *net, int topK, float margin, float minScore, float minRatio,
+                      int32_t *pairs, int capacity, int *count, int64_t *stats);
+/* Diagnostic: the last learned job's candidate table, n x topK right indices
+ * (-1 for none), n <= the left count of that job; *topK receives K. */
+int v3dm_learned_candidates(v3dm_ctx *ctx, int n, int32_t *cand, int *topK);
 
 /* Diagnostic, no counterpart in the reference: per left feature k < n of the
  * last completed match job, the best and second-best score among the

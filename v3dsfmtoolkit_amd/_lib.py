@@ -95,9 +95,9 @@ _SIGNATURES = {
     "v3dm_mlp_train": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float,
                                       ctypes.c_uint64, _f32_p]),
     "v3dm_mlp_embed": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _u8_p, _f32_p]),
-    "v3dm_learned_find": (ctypes.c_int, [_ctx_p, ctypes.c_void_p, ctypes.c_float, ctypes.c_float, ctypes.c_float,
-                                         _i32_p, ctypes.c_int, _c_int_p, _i64_p]),
-    "v3dm_learned_candidates": (ctypes.c_int, [_ctx_p, ctypes.c_int, _i32_p]),
+    "v3dm_learned_find": (ctypes.c_int, [_ctx_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_float,
+                                         ctypes.c_float, _i32_p, ctypes.c_int, _c_int_p, _i64_p]),
+    "v3dm_learned_candidates": (ctypes.c_int, [_ctx_p, ctypes.c_int, _i32_p, _c_int_p]),
 }
 
 _lib = None

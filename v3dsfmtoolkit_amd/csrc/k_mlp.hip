@@ -259,7 +259,155 @@ __global__ void k_sample(int B, uint64_t seed, uint64_t step, int64_t n, int64_t
    idx[i] = (int64_t)(z % (uint64_t)n);
 }
 
+// ---- fused inference (MFMA) ---------------------------------------------------
+//
+// One wave embeds 32 rows, computing the transposed layers so that every lane
+// owns one data row as an MFMA column:
+//   layer 1, a1^T = W1r x^T on v_mfma_f32_32x32x16_bf16: A = W1r (hidden
+//   units as rows, BF16), B = x^T (x = u / 64 is exact in BF16, so every
+//   product is exact; FP32 accumulation).  Accumulator register v of output
+//   block n in lane l holds unit 32n + 8(v/4) + 4(l/32) + v%4 of row l%32.
+//   layer 2, z^T = W2r h^T on v_mfma_f32_32x32x2_f32 (FP32 throughout): in
+//   step t = 16n + v the lane half h supplies k = h, paired with unit
+//   U = 32n + 8(v/4) + 4h + v%4 -- exactly the h value that lane already
+//   holds in register (n, v) -- and A supplies W2r[o][U].  The sum over k
+//   does not depend on which unit a step assigns to which k slot.
+// The L2 norm of a row combines the two lane halves (outputs 8a + b and
+// 8a + 4 + b).  Same arithmetic as the training forward pass up to the order
+// of the FP32 sums.
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef float v16f __attribute__((ext_vector_type(16)));
+typedef int v4i_ __attribute__((ext_vector_type(4)));
+
+// W1bf[u][k] = W1r[u][k] as BF16 bits (exact: already rounded);
+// W2T[u][o] = W2r[o][u].
+__global__ void k_embed_weights(const float *__restrict__ Pr, uint16_t *__restrict__ W1bf, float *__restrict__ W2T)
+{
+   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+   if (i < 256 * 128) W1bf[i] = (uint16_t)(__float_as_uint(Pr[i]) >> 16);
+   const int w2 = 256 * 128 + 256;
+   if (i < 64 * 256)
+   {
+      const int o = i / 256, u = i % 256;
+      W2T[u * 64 + o] = Pr[w2 + i];
+   }
+}
+
+__global__ __launch_bounds__(256) void k_embed_mfma(int n, const uint8_t *__restrict__ U8,
+                                                    const uint16_t *__restrict__ W1bf, const float *__restrict__ b1,
+                                                    const float *__restrict__ W2T, const float *__restrict__ b2,
+                                                    float *__restrict__ outF, uint16_t *__restrict__ outBF)
+{
+   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+   const int half = lane >> 5, lj = lane & 31;
+   const int row0 = (blockIdx.x * 4 + wave) * 32;
+   if (row0 >= n) return;
+   const int row = row0 + lj;
+   const bool live = row < n;
+   // B operand of layer 1: x^T fragments, k in [16s + 8h, 16s + 8h + 8)
+   v4i_ X[8];
+#pragma unroll
+   for (int s = 0; s < 8; ++s)
+   {
+      uint2 u = make_uint2(0, 0);
+      if (live) u = *reinterpret_cast<const uint2 *>(U8 + (size_t)row * 128 + 16 * s + 8 * half);
+      uint32_t w[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+      {
+         const uint32_t lo = ((e < 2 ? u.x : u.y) >> (16 * (e & 1))) & 0xFFu;
+         const uint32_t hi = ((e < 2 ? u.x : u.y) >> (16 * (e & 1) + 8)) & 0xFFu;
+         w[e] = (__float_as_uint((float)lo * 0.015625f) >> 16) | (__float_as_uint((float)hi * 0.015625f) & 0xFFFF0000u);
+      }
+      X[s] = v4i_{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
+   }
+   // per hidden block nb: layer-1 MFMAs, GELU, then that block's layer-2
+   // steps, so only one block of hidden values is live at a time
+   const v16f zero = {};
+   v16f z[2] = {zero, zero};
+#pragma unroll 1
+   for (int nb = 0; nb < 8; ++nb)
+   {
+      float w2[2][16];
+#pragma unroll
+      for (int v = 0; v < 16; ++v)
+      {
+         const int u = 32 * nb + 8 * (v >> 2) + 4 * half + (v & 3);
+#pragma unroll
+         for (int ob = 0; ob < 2; ++ob) w2[ob][v] = W2T[u * 64 + 32 * ob + lj];
+      }
+      const uint16_t *wrow = W1bf + (size_t)(32 * nb + lj) * 128 + 8 * half;
+      v4i_ W[8];
+#pragma unroll
+      for (int s = 0; s < 8; ++s) W[s] = *reinterpret_cast<const v4i_ *>(wrow + 16 * s);
+      v16f a = zero;
+#pragma unroll
+      for (int s = 0; s < 8; ++s)
+         a = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(v8bf, W[s]), __builtin_bit_cast(v8bf, X[s]), a,
+                                                    0, 0, 0);
+      // register v is unit 32 nb + 8(v/4) + 4 half + v%4
+#pragma unroll
+      for (int v = 0; v < 16; ++v)
+      {
+         const int u = 32 * nb + 8 * (v >> 2) + 4 * half + (v & 3);
+         const float hv = gelu(a[v] + b1[u]);
+#pragma unroll
+         for (int ob = 0; ob < 2; ++ob) z[ob] = __builtin_amdgcn_mfma_f32_32x32x2f32(w2[ob][v], hv, z[ob], 0, 0, 0);
+      }
+   }
+   // bias, norm over the 64 outputs of the row (this half holds 32 of them)
+   float ss = 0.0f;
+#pragma unroll
+   for (int ob = 0; ob < 2; ++ob)
+#pragma unroll
+      for (int v = 0; v < 16; ++v)
+      {
+         const int o = 32 * ob + 8 * (v >> 2) + 4 * half + (v & 3);
+         z[ob][v] += b2[o];
+         ss = fmaf(z[ob][v], z[ob][v], ss);
+      }
+   ss += __shfl_xor(ss, 32, 64);
+   const float nrm = sqrtf(ss + 1e-12f);
+   if (!live) return;
+#pragma unroll
+   for (int ob = 0; ob < 2; ++ob)
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+      {
+         const int o = 32 * ob + 8 * a + 4 * half;
+         float e[4];
+#pragma unroll
+         for (int b = 0; b < 4; ++b) e[b] = z[ob][4 * a + b] / nrm;
+         if (outF) *reinterpret_cast<float4 *>(outF + (size_t)row * 64 + o) = make_float4(e[0], e[1], e[2], e[3]);
+         if (outBF)
+         {
+            uint32_t h2[2];
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+            {
+               const uint32_t x0 = __float_as_uint(e[2 * b]), x1 = __float_as_uint(e[2 * b + 1]);
+               h2[b] = ((x0 + 0x7FFFu + ((x0 >> 16) & 1u)) >> 16) | (((x1 + 0x7FFFu + ((x1 >> 16) & 1u)) >> 16) << 16);
+            }
+            *reinterpret_cast<uint2 *>(outBF + (size_t)row * 64 + o) = make_uint2(h2[0], h2[1]);
+         }
+      }
+}
+
 // ---- launchers --------------------------------------------------------------
+
+hipError_t launch_embed_weights(const float *Pr, uint16_t *W1bf, float *W2T, hipStream_t st)
+{
+   hipLaunchKernelGGL(k_embed_weights, dim3(128), dim3(256), 0, st, Pr, W1bf, W2T);
+   return hipGetLastError();
+}
+
+hipError_t launch_embed_mfma(int n, const uint8_t *U8, const uint16_t *W1bf, const float *b1, const float *W2T,
+                             const float *b2, float *outF, uint16_t *outBF, hipStream_t st)
+{
+   if (n <= 0) return hipSuccess;
+   hipLaunchKernelGGL(k_embed_mfma, dim3((n + 127) / 128), dim3(256), 0, st, n, U8, W1bf, b1, W2T, b2, outF, outBF);
+   return hipGetLastError();
+}
 
 hipError_t launch_gemm_f32(bool ta, bool tb, int M, int N, int K, const float *A, int lda, const float *B, int ldb,
                            float *C, int ldc, const float *bias, float alpha, bool accumulate, hipStream_t st)

@@ -27,14 +27,18 @@ hipError_t launch_adam(int n, float *W, const float *g, float *m, float *v, floa
 hipError_t launch_round_copy(int n, const float *W, float *Wr, bool round, hipStream_t st);
 hipError_t launch_u8_rows_to_x(int n, const uint8_t *u, const int64_t *idx, float *x, hipStream_t st);
 hipError_t launch_sample(int B, uint64_t seed, uint64_t step, int64_t n, int64_t *idx, hipStream_t st);
-int emb_topk_chunks(int nR);
-hipError_t launch_emb_topk(int nQ, int nR, const uint16_t *EQ, const uint16_t *ER, int nChunks, float *partS,
-                           int *partI, int *cand, float *candS, hipStream_t st);
+hipError_t launch_embed_weights(const float *Pr, uint16_t *W1bf, float *W2T, hipStream_t st);
+hipError_t launch_embed_mfma(int n, const uint8_t *U8, const uint16_t *W1bf, const float *b1, const float *W2T,
+                             const float *b2, float *outF, uint16_t *outBF, hipStream_t st);
+int emb_topk_chunks(int nR, int K);
+size_t emb_topk_scratch_bytes(int nQ, int nChunks);
+hipError_t launch_emb_topk(int nQ, int nR, int K, const uint16_t *EQ, const uint16_t *ER, int nChunks, void *scratch,
+                           int *cand, float *candS, int *ovf, hipStream_t st);
 hipError_t launch_cand_insert(int nL, int nR, int K, const int *cand, const int8_t *L, const int8_t *R,
                               uint64_t *top1L, uint64_t *top2L, uint64_t *top1R, uint64_t *top2R,
                               const uint32_t *epoch, hipStream_t st);
 hipError_t launch_flag(int n, int isRight, const uint64_t *top1, const uint64_t *top2, const uint32_t *epoch,
-                       float margin, int *list, int *count, hipStream_t st);
+                       float margin, const int *force, int *list, int *count, hipStream_t st);
 hipError_t launch_gather_rows(int n, int nPad, const int *idx, const int8_t *src, const int32_t *srcSum, int8_t *dst,
                               int32_t *dstSum, hipStream_t st);
 } // namespace v3dm
@@ -44,8 +48,7 @@ namespace {
 constexpr int kIn = 128, kHid = 256, kOut = 64;
 constexpr int kOffW1 = 0, kOffB1 = kHid * kIn, kOffW2 = kOffB1 + kHid, kOffB2 = kOffW2 + kOut * kHid;
 constexpr int kParams = kOffB2 + kOut; // 49472
-constexpr int kInferRows = 16384;      // rows per embedding chunk
-constexpr int kCandK = 16;
+constexpr int kMaxCandK = 32; // candidates per query: 16 or 32
 
 } // namespace
 
@@ -69,7 +72,9 @@ struct v3dm_mlp
    float *dDE = nullptr, *dDZ = nullptr, *dDH = nullptr, *dDA1 = nullptr;
    float *dLossHist = nullptr;
    int lossHistCap = 0;
-   float *iX = nullptr, *iA1 = nullptr, *iH = nullptr, *iZ = nullptr, *iE = nullptr;
+   float *iE = nullptr;
+   uint16_t *dW1bf = nullptr; // inference copies of the forward weights
+   float *dW2T = nullptr;
    uint8_t *iU8 = nullptr;
    int iU8Cap = 0;
 };
@@ -81,16 +86,17 @@ struct LearnedState
 {
    int capL = 0, capR = 0;
    uint16_t *embL = nullptr, *embR = nullptr;
-   float *partS = nullptr;
-   int *partI = nullptr;
-   int partCap = 0;
+   void *scratch = nullptr; // retrieval sweeps: group maxima, thresholds, candidate lists
+   size_t scratchBytes = 0;
    int *cand = nullptr;
+   int *ovf = nullptr;      // per left: candidate list overflowed
    int *flagL = nullptr, *flagR = nullptr, *dCount = nullptr, *hCount = nullptr;
    int8_t *gI8 = nullptr;
    int32_t *gSum = nullptr;
    int gCap = 0;
    PairDesc *dPairFb = nullptr; // the two fallback scans' pair descriptors
    int lastNL = 0;
+   int lastK = 0;
 };
 
 void freeLearned(v3dm_ctx *c)
@@ -99,9 +105,9 @@ void freeLearned(v3dm_ctx *c)
    if (!s) return;
    dfree(s->embL);
    dfree(s->embR);
-   dfree(s->partS);
-   dfree(s->partI);
+   dfree(s->scratch);
    dfree(s->cand);
+   dfree(s->ovf);
    dfree(s->flagL);
    dfree(s->flagR);
    dfree(s->dCount);
@@ -142,8 +148,9 @@ void freeMlp(v3dm_mlp *m)
 {
    for (float **p : {&m->dP, &m->dPr, &m->dG, &m->dM, &m->dV, &m->dX, &m->dA1, &m->dH, &m->dZ, &m->dE, &m->dN,
                      &m->dS, &m->dGs, &m->dLseR, &m->dLseC, &m->dLoss, &m->dDE, &m->dDZ, &m->dDH, &m->dDA1,
-                     &m->dLossHist, &m->iX, &m->iA1, &m->iH, &m->iZ, &m->iE})
+                     &m->dLossHist, &m->iE, &m->dW2T})
       dfree(*p);
+   dfree(m->dW1bf);
    dfree(m->dTL);
    dfree(m->dTR);
    dfree(m->dBL);
@@ -241,29 +248,20 @@ int adamStep(v3dm_mlp *m, float lr)
    return V3DM_OK;
 }
 
-// Embeddings of n uint8 rows (device), on stream st: FP32 rows to outF and/or
-// BF16 rows to outBF (rows n..padTo-1 of outBF are zeroed).
+// Embeddings of n uint8 rows (device), on stream st, by the fused MFMA
+// kernel: FP32 rows to outF and/or BF16 rows to outBF (rows n..padTo-1 of
+// outBF are zeroed).
 int embedRows(v3dm_mlp *m, hipStream_t st, int n, const uint8_t *dU8, float *outF, uint16_t *outBF, int padTo)
 {
-   if (!m->iX)
+   if (!m->dW1bf)
    {
-      MLPCHK(m, hipMalloc((void **)&m->iX, sizeof(float) * (size_t)kInferRows * kIn));
-      MLPCHK(m, hipMalloc((void **)&m->iA1, sizeof(float) * (size_t)kInferRows * kHid));
-      MLPCHK(m, hipMalloc((void **)&m->iH, sizeof(float) * (size_t)kInferRows * kHid));
-      MLPCHK(m, hipMalloc((void **)&m->iZ, sizeof(float) * (size_t)kInferRows * kOut));
+      MLPCHK(m, hipMalloc((void **)&m->dW1bf, sizeof(uint16_t) * kHid * kIn));
+      MLPCHK(m, hipMalloc((void **)&m->dW2T, sizeof(float) * kHid * kOut));
    }
-   const float *W1 = m->dPr + kOffW1, *b1 = m->dPr + kOffB1, *W2 = m->dPr + kOffW2, *b2 = m->dPr + kOffB2;
-   for (int r0 = 0; r0 < n; r0 += kInferRows)
-   {
-      const int rows = std::min(kInferRows, n - r0);
-      MLPCHK(m, launch_u8_rows_to_x(rows, dU8 + (size_t)r0 * kIn, nullptr, m->iX, st));
-      MLPCHK(m, launch_gemm_f32(false, true, rows, kHid, kIn, m->iX, kIn, W1, kIn, m->iA1, kHid, b1, 1.0f, false, st));
-      MLPCHK(m, launch_gelu_fwd((int64_t)rows * kHid, m->iA1, m->iH, st));
-      MLPCHK(m, launch_gemm_f32(false, true, rows, kOut, kHid, m->iH, kHid, W2, kHid, m->iZ, kOut, b2, 1.0f, false, st));
-      MLPCHK(m, launch_l2n_fwd(rows, m->iZ, outF ? outF + (size_t)r0 * kOut : nullptr, nullptr,
-                               outBF ? outBF + (size_t)r0 * kOut : nullptr, st));
-   }
-   if (outBF && padTo > n) MLPCHK(m, hipMemsetAsync(outBF + (size_t)n * kOut, 0, sizeof(uint16_t) * (size_t)(padTo - n) * kOut, st));
+   MLPCHK(m, launch_embed_weights(m->dPr, m->dW1bf, m->dW2T, st));
+   MLPCHK(m, launch_embed_mfma(n, dU8, m->dW1bf, m->dPr + kOffB1, m->dW2T, m->dPr + kOffB2, outF, outBF, st));
+   if (outBF && padTo > n)
+      MLPCHK(m, hipMemsetAsync(outBF + (size_t)n * kOut, 0, sizeof(uint16_t) * (size_t)(padTo - n) * kOut, st));
    return V3DM_OK;
 }
 
@@ -464,11 +462,12 @@ int v3dm_mlp_embed(v3dm_mlp *m, int n, const uint8_t *desc, float *out)
    return V3DM_OK;
 }
 
-int v3dm_learned_find(v3dm_ctx *c, v3dm_mlp *m, float margin, float minScore, float minRatio, int32_t *pairs,
-                      int capacity, int *count, int64_t *stats)
+int v3dm_learned_find(v3dm_ctx *c, v3dm_mlp *m, int topK, float margin, float minScore, float minRatio,
+                      int32_t *pairs, int capacity, int *count, int64_t *stats)
 {
    if (!c) return V3DM_ERR_ARG;
    if (!m) return fail(c, V3DM_ERR_ARG, "learned_find: network is NULL");
+   if (topK != 16 && topK != 32) return fail(c, V3DM_ERR_ARG, "learned_find: topK must be 16 or 32");
    if (capacity < 0) return fail(c, V3DM_ERR_ARG, "learned_find: negative capacity");
    if (!c->allocated) return fail(c, V3DM_ERR_STATE, "learned_find: allocate() has not been called");
    if (m->device != c->device) return fail(c, V3DM_ERR_ARG, "learned_find: network and matcher on different devices");
@@ -497,6 +496,7 @@ int v3dm_learned_find(v3dm_ctx *c, v3dm_mlp *m, float margin, float minScore, fl
       dfree(s->embL);
       dfree(s->embR);
       dfree(s->cand);
+      dfree(s->ovf);
       dfree(s->flagL);
       dfree(s->flagR);
       dfree(s->gI8);
@@ -504,7 +504,8 @@ int v3dm_learned_find(v3dm_ctx *c, v3dm_mlp *m, float margin, float minScore, fl
       s->capL = s->capR = 0;
       HIPCHK(c, hipMalloc((void **)&s->embL, sizeof(uint16_t) * (size_t)capL * kOut));
       HIPCHK(c, hipMalloc((void **)&s->embR, sizeof(uint16_t) * (size_t)capR * kOut));
-      HIPCHK(c, hipMalloc((void **)&s->cand, sizeof(int) * (size_t)capL * kCandK));
+      HIPCHK(c, hipMalloc((void **)&s->cand, sizeof(int) * (size_t)capL * kMaxCandK));
+      HIPCHK(c, hipMalloc((void **)&s->ovf, sizeof(int) * (size_t)capL));
       HIPCHK(c, hipMalloc((void **)&s->flagL, sizeof(int) * (size_t)capL));
       HIPCHK(c, hipMalloc((void **)&s->flagR, sizeof(int) * (size_t)capR));
       const int g = std::max(capL, capR);
@@ -519,16 +520,14 @@ int v3dm_learned_find(v3dm_ctx *c, v3dm_mlp *m, float margin, float minScore, fl
       HIPCHK(c, hipHostMalloc((void **)&s->hCount, 2 * sizeof(int), hipHostMallocDefault));
       HIPCHK(c, hipMalloc((void **)&s->dPairFb, 2 * sizeof(PairDesc)));
    }
-   const int nChunks = emb_topk_chunks(nR);
-   const int needPart = std::max(nL, 1) * nChunks * kCandK;
-   if (needPart > s->partCap)
+   const int nChunks = emb_topk_chunks(nR, topK);
+   const size_t needScratch = emb_topk_scratch_bytes(nL, nChunks);
+   if (needScratch > s->scratchBytes)
    {
-      dfree(s->partS);
-      dfree(s->partI);
-      s->partCap = 0;
-      HIPCHK(c, hipMalloc((void **)&s->partS, sizeof(float) * (size_t)needPart));
-      HIPCHK(c, hipMalloc((void **)&s->partI, sizeof(int) * (size_t)needPart));
-      s->partCap = needPart;
+      dfree(s->scratch);
+      s->scratchBytes = 0;
+      HIPCHK(c, hipMalloc(&s->scratch, needScratch));
+      s->scratchBytes = needScratch;
    }
    const PairDesc &pd = c->hPairUploaded;
    int64_t nFlag[2] = {0, 0};
@@ -540,13 +539,14 @@ int v3dm_learned_find(v3dm_ctx *c, v3dm_mlp *m, float margin, float minScore, fl
       if (rc) return fail(c, rc, "learned_find: %s", m->err.c_str());
       rc = embedRows(m, st, nR, c->R.dRawU8, nullptr, s->embR, roundUp(nR, 32));
       if (rc) return fail(c, rc, "learned_find: %s", m->err.c_str());
-      HIPCHK(c, launch_emb_topk(nL, nR, s->embL, s->embR, nChunks, s->partS, s->partI, s->cand, nullptr, st));
-      HIPCHK(c, launch_cand_insert(nL, nR, kCandK, s->cand, c->L.dI8, c->R.dI8, c->L.dTop1, c->L.dTop2, c->R.dTop1,
+      HIPCHK(c, launch_emb_topk(nL, nR, topK, s->embL, s->embR, nChunks, s->scratch, s->cand, nullptr, s->ovf, st));
+      HIPCHK(c, launch_cand_insert(nL, nR, topK, s->cand, c->L.dI8, c->R.dI8, c->L.dTop1, c->L.dTop2, c->R.dTop1,
                                    c->R.dTop2, c->dEpoch, st));
       // 4. margin flags
       HIPCHK(c, hipMemsetAsync(s->dCount, 0, 2 * sizeof(int), st));
-      HIPCHK(c, launch_flag(nL, 0, c->L.dTop1, c->L.dTop2, c->dEpoch, margin, s->flagL, s->dCount, st));
-      HIPCHK(c, launch_flag(nR, 1, c->R.dTop1, c->R.dTop2, c->dEpoch, margin, s->flagR, s->dCount + 1, st));
+      HIPCHK(c, launch_flag(nL, 0, c->L.dTop1, c->L.dTop2, c->dEpoch, margin, s->ovf, s->flagL, s->dCount, st));
+      HIPCHK(c, launch_flag(nR, 1, c->R.dTop1, c->R.dTop2, c->dEpoch, margin, nullptr, s->flagR, s->dCount + 1,
+                            st));
       HIPCHK(c, hipMemcpyAsync(s->hCount, s->dCount, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
       rc = syncStream(c);
       if (rc) return rc;
@@ -621,11 +621,12 @@ int v3dm_learned_find(v3dm_ctx *c, v3dm_mlp *m, float margin, float minScore, fl
    c->lastKernel = V3DM_KERNEL_I8;
    c->lastNL = nL;
    s->lastNL = nL;
+   s->lastK = topK;
    if (stats)
    {
       stats[0] = nL;
       stats[1] = nR;
-      stats[2] = (int64_t)nL * kCandK;
+      stats[2] = (int64_t)nL * topK;
       stats[3] = nFlag[0];
       stats[4] = nFlag[1];
       stats[5] = nFlag[0] * nR + (int64_t)nL * nFlag[1];
@@ -634,7 +635,7 @@ int v3dm_learned_find(v3dm_ctx *c, v3dm_mlp *m, float margin, float minScore, fl
    return fetch(c, pairs, capacity, count);
 }
 
-int v3dm_learned_candidates(v3dm_ctx *c, int n, int32_t *cand)
+int v3dm_learned_candidates(v3dm_ctx *c, int n, int32_t *cand, int *topK)
 {
    if (!c) return V3DM_ERR_ARG;
    if (!c->learned || n < 0 || n > c->learned->lastNL || (n > 0 && !cand))
@@ -642,7 +643,9 @@ int v3dm_learned_candidates(v3dm_ctx *c, int n, int32_t *cand)
    HIPCHK(c, hipSetDevice(c->device));
    int rc = syncStream(c);
    if (rc) return rc;
-   if (n > 0) HIPCHK(c, hipMemcpy(cand, c->learned->cand, sizeof(int32_t) * (size_t)n * kCandK, hipMemcpyDeviceToHost));
+   const int K = c->learned->lastK;
+   if (topK) *topK = K;
+   if (n > 0) HIPCHK(c, hipMemcpy(cand, c->learned->cand, sizeof(int32_t) * (size_t)n * K, hipMemcpyDeviceToHost));
    return V3DM_OK;
 }
 

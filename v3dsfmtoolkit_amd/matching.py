@@ -129,22 +129,28 @@ class SIFTMatching:
 
     # -- learned proposal path ----------------------------------------------------
     def learnedFind(self, net: "EmbeddingNet", margin: float = 0.05, minScore: float = 0.9,
-                    minRatio: float = 0.8) -> tuple[list[tuple[int, int]], dict]:
+                    minRatio: float = 0.8, topK: int = 16, asArray: bool = False):
         """Matches the current pair through the learned proposal path (both
-        sides uploaded as uint8).  Returns (pairs, stats)."""
+        sides uploaded as uint8).  Returns (pairs, stats); pairs as an (n, 2)
+        int32 array when asArray."""
         pairs, count = self._buffer()
         st = np.zeros(6, np.int64)
-        check(self._lib.v3dm_learned_find(self._ctx, net._h, margin, minScore, minRatio, ptr(pairs, ctypes.c_int32),
+        check(self._lib.v3dm_learned_find(self._ctx, net._h, topK, margin, minScore, minRatio,
+                                          ptr(pairs, ctypes.c_int32),
                                           self._cap, ctypes.byref(count), ptr(st, ctypes.c_int64)), self._ctx)
         keys = ["n_left", "n_right", "candidates", "flagged_left", "flagged_right", "fallback_pairs"]
-        return _as_list(pairs, count.value), {k: int(x) for k, x in zip(keys, st)}
+        stats = {k: int(x) for k, x in zip(keys, st)}
+        if asArray:
+            return pairs[:count.value].copy(), stats
+        return _as_list(pairs, count.value), stats
 
     def learnedCandidates(self) -> np.ndarray:
-        """(n_left, 16) candidate right indices of the last learned job."""
+        """(n_left, K) candidate right indices of the last learned job."""
         n = self.counts()[0]
-        out = np.empty((n, 16), np.int32)
-        check(self._lib.v3dm_learned_candidates(self._ctx, n, ptr(out, ctypes.c_int32)), self._ctx)
-        return out
+        out = np.empty((n, 32), np.int32)
+        k = ctypes.c_int(0)
+        check(self._lib.v3dm_learned_candidates(self._ctx, n, ptr(out, ctypes.c_int32), ctypes.byref(k)), self._ctx)
+        return out.reshape(-1)[:n * k.value].reshape(n, k.value).copy()
 
     # -- many pairs per job (v3dm_batch_*) ---------------------------------------
     def setImages(self, images) -> list[int]:
