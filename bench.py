@@ -47,7 +47,7 @@ def cpu_baseline(L, R, min_score, min_ratio, budget_s=20.0):
             "seconds_per_pair": med}, tab
 
 
-TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_i8_scan_8192x8192_traffic.md")
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_i8_scan_8192x8192_traffic.md")  # per scan variant
 
 
 def measured_traffic(kernel_name):
@@ -88,6 +88,10 @@ def main():
                     help="c2: single pair 8192^2 (default line); c3: 256 uneven pairs in one batch (+ float "
                          "variant); c4: learned path on 64 pairs of 16384^2; c5: single pair 65536^2")
     ap.add_argument("--pairs", type=int, default=0, help="number of pairs for c3 / c4 (0: the workload's own)")
+    ap.add_argument("--scan-variant", type=int, default=-1,
+                    help="single-pair int8 scan variant (-1: the library measures the fastest)")
+    ap.add_argument("--profile-variants", action="store_true",
+                    help="after the measurement, run 20 jobs with each scan variant (for profiler passes)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -117,6 +121,7 @@ def run_c2(args, v):
     m.setRightFeatures(n, R)
     kernel = v.KERNEL_I8 if args.kernel == "i8" else v.KERNEL_PLAIN
     m.setKernel(kernel)
+    m.setScanVariant(args.scan_variant)
 
     # the measured kernel must agree with the reference-design kernel
     ref_list = None
@@ -142,15 +147,20 @@ def run_c2(args, v):
         m.findAsync(min_score, min_ratio)
     m.synchronize()
 
+    # kernel times: HIP events bound to the scan dispatch of every job
     m.profileEnable(True)
+    for _ in range(min(args.steps, 1000)):
+        m.findAsync(min_score, min_ratio)
+    m.synchronize()
+    scan_ms, sel_ms, jobs = m.profileRead()
+    m.profileEnable(False)
+    # throughput: the same jobs with no events in the stream
     m.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         m.findAsync(min_score, min_ratio)
     m.synchronize()
     t1 = time.perf_counter()
-    scan_ms, sel_ms, jobs = m.profileRead()
-    m.profileEnable(False)
     final = m.fetchMatches()
     assert final == got
 
@@ -160,9 +170,10 @@ def run_c2(args, v):
     sel_avg_s = sel_ms / 1e3 / max(jobs, 1)
     if kernel == v.KERNEL_I8:
         achieved = pairs * 256 / scan_avg_s / 1e12
-        roof = {"bound": "mfma", "kernel": "k_i8_scan", "achieved": achieved, "peak": INT8_PEAK_TOPS,
+        roof = {"bound": "mfma", "kernel": _scan_name(v, m.lastScanVariant()), "achieved": achieved,
+                "peak": INT8_PEAK_TOPS,
                 "unit": "TFLOP/s", "frac": achieved / INT8_PEAK_TOPS,
-                "traffic": measured_traffic("k_i8_scan") if n == 8192 else None,
+                "traffic": measured_traffic(_scan_name(v, m.lastScanVariant())) if n == 8192 else None,
                 "ops_per_pair": 256, "launch_avg_us": scan_avg_s * 1e6}
     else:
         achieved = pairs * 256 / scan_avg_s / 1e12
@@ -185,7 +196,9 @@ def run_c2(args, v):
         "data": "synthetic",
         "config": {"workload": f"single pair {n}x{n} uint8 SIFT-like descriptors, 40% noisy copies (sigma 0.05), "
                                "minScore 0.8, minRatio 0.8, mutual check",
-                   "n_left": n, "n_right": n, "kernel": args.kernel, "matches": len(got)},
+                   "n_left": n, "n_right": n, "kernel": args.kernel, "matches": len(got),
+                   "scan_variant": (list(v.scan_variants()[m.lastScanVariant()])
+                                    if kernel == v.KERNEL_I8 else None)},
         "image_pairs_per_s": 1.0 / wall,
         "select_avg_us": sel_avg_s * 1e6,
         "roofline": roof,
@@ -200,10 +213,23 @@ def run_c2(args, v):
         from oracle import oracle as O
         cb["agrees_with_gpu"] = O.table_to_pairs(tab) == got
         out["cpu_baseline"] = cb
+    if args.profile_variants and kernel == v.KERNEL_I8:
+        for var in range(len(v.scan_variants())):
+            m.setScanVariant(var)
+            for _ in range(20):
+                m.findAsync(min_score, min_ratio)
+            m.synchronize()
+            assert m.fetchMatches() == got
     m.close()
     if args.copy_probe_mib > 0:
         copy_probe(args.copy_probe_mib)
     print(json.dumps(out), flush=True)
+
+
+def _scan_name(v, variant):
+    """Kernel name of a scan variant as rocprofv3 prints it, e.g. k_i8_scan<8, 512>."""
+    w, _, tn = v.scan_variants()[variant]
+    return f"k_i8_scan<{w}, {tn}>"
 
 
 def _common(metric, value, unit, steps, warmup, wall, dtype, config, higher=True):

@@ -118,6 +118,14 @@ int v3dm_run_guided_homography_matching(v3dm_ctx *ctx, const double H[9], float 
                                         const float *leftPos, const float *rightPos, float minScore,
                                         float minRatio, int32_t *pairs, int capacity, int *count);
 
+/* Scan tile variant of the single-pair int8 path (0 .. v3dm_scan_variant_count()-1),
+ * or -1 (default): the fastest, measured once per shape bucket (each side's
+ * count rounded up to a power of two) on the first job of that bucket and
+ * cached in the context.  No counterpart in the reference. */
+int v3dm_set_scan_variant(v3dm_ctx *ctx, int variant);
+/* The scan variant the last single-pair int8 job ran (-1 before any). */
+int v3dm_get_last_scan_variant(const v3dm_ctx *ctx, int *variant);
+
 /* Kernel selection (V3DM_KERNEL_*) and the kernel the last match call ran. */
 int v3dm_set_kernel(v3dm_ctx *ctx, int kernel);
 int v3dm_get_last_kernel(const v3dm_ctx *ctx, int *kernel);

@@ -278,9 +278,10 @@ int chooseKernel(v3dm_ctx *c, int *kernel)
 }
 
 // Enqueue the i8 job for the current pair (masks already uploaded).
-int enqueueI8(v3dm_ctx *c, float minScore, float minRatio)
+// One int8 job with scan variant v; marks: record the profiling events.
+int enqueueI8Job(v3dm_ctx *c, int v, float minScore, float minRatio, bool marks)
 {
-   int rc = uploadPair(c, 0);
+   int rc = uploadPair(c, v);
    if (rc) return rc;
    rc = useKeyFormat(c, 0);
    if (rc) return rc;
@@ -305,19 +306,19 @@ int enqueueI8(v3dm_ctx *c, float minScore, float minRatio)
    a.surv = c->dSurv;
    a.survCount = c->dSurvCount;
    a.survCap = c->survCap;
-   a.waves = scan_variant_waves(0);
-   a.tileM = scan_variant_tile_m(0);
-   a.tileN = scan_variant_tile_n(0);
+   a.waves = scan_variant_waves(v);
+   a.tileM = scan_variant_tile_m(v);
+   a.tileN = scan_variant_tile_n(v);
    a.persistent = 0;
    a.tileCounter = c->dEpoch + 2;
-   hipEvent_t e0, e1;
-   if ((rc = profKernelEvents(c, &e0, &e1))) return rc;
+   hipEvent_t e0 = nullptr, e1 = nullptr;
+   if (marks && (rc = profKernelEvents(c, &e0, &e1))) return rc;
    if (a.nTiles <= 0 && e0)
    {
       HIPCHK(c, hipEventRecord(e0, c->stream));
       HIPCHK(c, hipEventRecord(e1, c->stream));
    }
-   HIPCHK(c, launch_i8_scan(0, a.nTiles, a, c->stream, e0, e1));
+   HIPCHK(c, launch_i8_scan(v, a.nTiles, a, c->stream, e0, e1));
    HIPCHK(c, launch_i8_insert(a, c->stream));
    FinalizeArgs f{};
    f.pairs = c->dPair;
@@ -334,9 +335,77 @@ int enqueueI8(v3dm_ctx *c, float minScore, float minRatio)
    f.minRatio = minRatio;
    f.matchOfLeft = c->dMatch;
    HIPCHK(c, launch_i8_finalize(f, c->stream));
-   if ((rc = profMark(c, 2))) return rc;
+   if (marks && (rc = profMark(c, 2))) return rc;
    c->hostEpoch += 1;
+   c->lastVariant = v;
    return V3DM_OK;
+}
+
+// Scan variant of the single-pair int8 path: the forced one, or the fastest
+// measured (whole jobs, HIP events) for this shape bucket -- each side's
+// count rounded up to a power of two -- cached in the context.
+int pickI8Variant(v3dm_ctx *c, float minScore, float minRatio, int *variant)
+{
+   *variant = 0;
+   if (c->scanVariant >= 0)
+   {
+      *variant = c->scanVariant;
+      return V3DM_OK;
+   }
+   if (c->L.n == 0 || c->R.n == 0) return V3DM_OK;
+   auto bucket = [](int n) {
+      uint64_t b = kRowPad;
+      while (b < (uint64_t)n) b <<= 1;
+      return b;
+   };
+   const uint64_t key = (bucket(c->L.n) << 32) | bucket(c->R.n);
+   for (const auto &t : c->tunedI8)
+      if (t.first == key)
+      {
+         *variant = t.second;
+         return V3DM_OK;
+      }
+   hipEvent_t t0, t1;
+   HIPCHK(c, hipEventCreate(&t0));
+   HIPCHK(c, hipEventCreate(&t1));
+   float best = 0.0f;
+   int rc = V3DM_OK;
+   // per variant: two warm-up jobs, then the best of three trials of four jobs
+   for (int v = 0; v < scan_variant_count() && rc == V3DM_OK; ++v)
+   {
+      for (int r = 0; r < 2 && rc == V3DM_OK; ++r) rc = enqueueI8Job(c, v, minScore, minRatio, false);
+      float vbest = 0.0f;
+      for (int trial = 0; trial < 3 && rc == V3DM_OK; ++trial)
+      {
+         if (hipEventRecord(t0, c->stream) != hipSuccess) rc = fail(c, V3DM_ERR_DEVICE, "autotune: event record");
+         for (int r = 0; r < 4 && rc == V3DM_OK; ++r) rc = enqueueI8Job(c, v, minScore, minRatio, false);
+         if (rc) break;
+         float ms = 0.0f;
+         if (hipEventRecord(t1, c->stream) != hipSuccess || hipEventSynchronize(t1) != hipSuccess ||
+             hipEventElapsedTime(&ms, t0, t1) != hipSuccess)
+            rc = fail(c, V3DM_ERR_DEVICE, "autotune: event timing");
+         else if (trial == 0 || ms < vbest)
+            vbest = ms;
+      }
+      if (rc == V3DM_OK && (v == 0 || vbest < best))
+      {
+         best = vbest;
+         *variant = v;
+      }
+   }
+   (void)hipEventDestroy(t0);
+   (void)hipEventDestroy(t1);
+   if (rc) return rc;
+   c->tunedI8.push_back({key, *variant});
+   return V3DM_OK;
+}
+
+int enqueueI8(v3dm_ctx *c, float minScore, float minRatio)
+{
+   int v = 0;
+   int rc = pickI8Variant(c, minScore, minRatio, &v);
+   if (rc) return rc;
+   return enqueueI8Job(c, v, minScore, minRatio, true);
 }
 
 // Float path: BF16 candidate scan, FP32 rescoring, finalize (masks already uploaded).
@@ -747,6 +816,22 @@ int v3dm_set_kernel(v3dm_ctx *c, int kernel)
    if (kernel < V3DM_KERNEL_AUTO || kernel > V3DM_KERNEL_BF16)
       return fail(c, V3DM_ERR_ARG, "set_kernel: unknown kernel %d", kernel);
    c->kernelSel = kernel;
+   return V3DM_OK;
+}
+
+int v3dm_set_scan_variant(v3dm_ctx *c, int variant)
+{
+   if (!c) return V3DM_ERR_ARG;
+   if (variant < -1 || variant >= scan_variant_count())
+      return fail(c, V3DM_ERR_ARG, "set_scan_variant: variant %d out of range", variant);
+   c->scanVariant = variant;
+   return V3DM_OK;
+}
+
+int v3dm_get_last_scan_variant(const v3dm_ctx *c, int *variant)
+{
+   if (!c || !variant) return V3DM_ERR_ARG;
+   *variant = c->lastVariant;
    return V3DM_OK;
 }
 

@@ -175,6 +175,9 @@ struct v3dm_ctx
    uint64_t *dSurv = nullptr;
    int32_t *dSurvCount = nullptr;
    int survCap = 64;
+   int scanVariant = -1;                                 // single-pair int8 scan variant, -1: measured
+   int lastVariant = -1;                                 // variant of the last single-pair int8 job
+   std::vector<std::pair<uint64_t, int>> tunedI8;       // shape bucket -> fastest variant
 
    // float path
    int bfVariant = 2;         // scan variant of the BF16 kernel

@@ -75,6 +75,15 @@ class SIFTMatching:
     def setKernel(self, kernel: int) -> None:
         check(self._lib.v3dm_set_kernel(self._ctx, int(kernel)), self._ctx)
 
+    def setScanVariant(self, variant: int) -> None:
+        """Single-pair int8 scan variant; -1 (default) measures the fastest per shape bucket."""
+        check(self._lib.v3dm_set_scan_variant(self._ctx, int(variant)), self._ctx)
+
+    def lastScanVariant(self) -> int:
+        k = ctypes.c_int(-1)
+        check(self._lib.v3dm_get_last_scan_variant(self._ctx, ctypes.byref(k)), self._ctx)
+        return k.value
+
     def lastKernel(self) -> int:
         k = ctypes.c_int(0)
         check(self._lib.v3dm_get_last_kernel(self._ctx, ctypes.byref(k)), self._ctx)
