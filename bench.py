@@ -251,10 +251,9 @@ def run_c3(args, v):
     images, fimages = [], []
     progress(f"generating {n_pairs} pairs")
     for i in range(n_pairs):
-        d = v.synth_pair(v.synth_seed(3, i), int(counts[i, 0]), int(counts[i, 1]), 0.4, 0.05, f32=i < 16)
+        d = v.synth_pair(v.synth_seed(3, i), int(counts[i, 0]), int(counts[i, 1]), 0.4, 0.05, f32=True)
         images += [d["L"], d["R"]]
-        if i < 16:
-            fimages.append((d["Lf"], d["Rf"]))
+        fimages += [d["Lf"], d["Rf"]]
     m = v.SIFTMatching(0)
     m.enableRoundingWarning(False)
     c32 = m.setImages(images)
@@ -264,6 +263,9 @@ def run_c3(args, v):
     m.batchSetVariant(best)
     # scan kernel time (events bound to the scan dispatch, eager jobs)
     m.batchSetGraph(False)
+    for _ in range(3):
+        m.batchMatchAsync(0.8, 0.8)
+    m.batchFetch()
     m.profileEnable(True)
     for _ in range(5):
         m.batchMatchAsync(0.8, 0.8)
@@ -315,40 +317,62 @@ def run_c3(args, v):
                                                      "oracle blocked rule; their tables equal the GPU's"}
     m.close()
     progress("batched int8 done; float variant")
-    # float variant: unquantised unit descriptors, single-pair BF16 path per pair
+    # float variant: the same pairs as unquantised unit floats, one batched job
+    # (BF16 candidates + FP32 rescoring); sampled pairs against the FP32 and
+    # FP64 oracles
     fm = v.SIFTMatching(0)
     fm.enableRoundingWarning(False)
-    fm.allocate(12000, 12000)
-    fm.setKernel(v.KERNEL_BF16)
-    f_wall, f_pairs, max_err, disagree, matches, exact_f32 = 0.0, 0, 0.0, 0, 0, True
-    for Lf, Rf in fimages:
-        nL, nR = len(Lf) - len(Lf) % 32, len(Rf) - len(Rf) % 32
-        fm.setLeftFeatures(nL, Lf)
-        fm.setRightFeatures(nR, Rf)
-        fm.findAsync(0.8, 0.8)
-        fm.synchronize()
-        t = time.perf_counter()
-        for _ in range(5):
-            fm.findAsync(0.8, 0.8)
-        fm.synchronize()
-        f_wall += (time.perf_counter() - t) / 5
-        f_pairs += nL * nR
-        got = fm.fetchMatches()
-        b, _, arg = fm.leftTop2()
-        have = arg >= 0
-        s64 = np.einsum("ij,ij->i", Lf[:nL][have].astype(np.float64), Rf[:nR][arg[have]].astype(np.float64))
-        max_err = max(max_err, float(np.abs(b[have] - s64).max()) if have.any() else 0.0)
-        l64 = set(O.table_to_pairs(O.find_f64(Lf[:nL], Rf[:nR], 0.8, 0.8)[2]))
+    fc32 = fm.setImages(fimages)
+    fm.setPairs([(2 * i, 2 * i + 1) for i in range(n_pairs)])
+    fbest, _ = fm.batchAutotune(0.8, 0.8, reps=2)
+    fm.batchSetVariant(fbest)
+    fm.batchSetGraph(True)
+    ftabs = fm.batchMatch(0.8, 0.8)
+    for _ in range(3):
+        fm.batchMatchAsync(0.8, 0.8)
+    fm.batchFetch()
+    t0 = time.perf_counter()
+    fsteps = max(steps // 2, 5)
+    for _ in range(fsteps):
+        fm.batchMatchAsync(0.8, 0.8)
+    fm.batchFetch()
+    f_wall = (time.perf_counter() - t0) / fsteps
+    fm.close()
+    exact_f32, disagree, matches, max_err = True, 0, 0, 0.0
+    sm = v.SIFTMatching(0)
+    sm.enableRoundingWarning(False)
+    sm.allocate(12000, 12000)
+    sm.setKernel(v.KERNEL_BF16)
+    for i in sample:
+        Lf, Rf = fimages[2 * i][:fc32[2 * i]], fimages[2 * i + 1][:fc32[2 * i + 1]]
+        exact_f32 &= np.array_equal(ftabs[i], O.find_f32(Lf, Rf, 0.8, 0.8)[2])
+        got = O.table_to_pairs(ftabs[i])
+        l64 = set(O.table_to_pairs(O.find_f64(Lf, Rf, 0.8, 0.8)[2]))
         disagree += len(set(got) ^ l64)
         matches += len(l64)
-        exact_f32 &= got == O.table_to_pairs(O.find_f32(Lf[:nL], Rf[:nR], 0.8, 0.8)[2])
-    fm.close()
-    out["float_variant"] = {"pairs": len(fimages), "descriptor_pairs_per_s": f_pairs / f_wall,
-                            "ms_per_pair": f_wall / len(fimages) * 1e3,
-                            "max_abs_score_error_vs_fp64": max_err,
-                            "list_disagreement_vs_fp64_rule": disagree, "fp64_rule_matches": matches,
-                            "equals_fp32_oracle": bool(exact_f32),
-                            "method": "BF16 MFMA candidates within the proven bound, FP32 rescoring, single-pair jobs"}
+        # the same rule's rescored scores (single-pair path) against FP64
+        sm.setLeftFeatures(len(Lf), Lf)
+        sm.setRightFeatures(len(Rf), Rf)
+        sm.findPutativeMatches(0.8, 0.8)
+        bsc, _, arg = sm.leftTop2()
+        have = arg >= 0
+        if have.any():
+            s64 = np.einsum("ij,ij->i", Lf[have].astype(np.float64), Rf[arg[have]].astype(np.float64))
+            max_err = max(max_err, float(np.abs(bsc[have] - s64).max()))
+    sm.close()
+    if not exact_f32:
+        print("bench.py: float batch tables differ from the FP32 oracle", file=sys.stderr)
+        sys.exit(1)
+    fw, _, ftn = v.scan_variants()[fbest]
+    out["float_variant"] = {"pairs": n_pairs, "descriptor_pairs_per_s": pairs_scored / f_wall,
+                            "image_pairs_per_s": n_pairs / f_wall, "ms_per_job": f_wall * 1e3,
+                            "variant": [fw, ftn], "graph": True,
+                            "sample_pairs": len(sample), "sample_equals_fp32_oracle": bool(exact_f32),
+                            "sample_max_abs_score_error_vs_fp64": max_err,
+                            "sample_list_disagreement_vs_fp64_rule": disagree,
+                            "sample_fp64_rule_matches": matches,
+                            "method": "one batched job: BF16 MFMA candidates within the proven bound, FP32 "
+                                      "rescoring, float-key finalize"}
     return out
 
 

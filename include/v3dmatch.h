@@ -169,6 +169,12 @@ int v3dm_profile_read(v3dm_ctx *ctx, double *scanMs, double *selectMs, int *jobs
  * pair list. */
 int v3dm_batch_set_images_u8(v3dm_ctx *ctx, int nImages, const int32_t *counts, const uint8_t *desc);
 
+/* As v3dm_batch_set_images_u8 for float descriptors (the reference's
+ * setLeftFeatures/setRightFeatures(int, const float*) input, e.g. SiftGPU's
+ * unit descriptors): the batch then runs the BF16 candidate scan + FP32
+ * rescoring, whose per-pair tables equal the plain FP32 rule's. */
+int v3dm_batch_set_images_f32(v3dm_ctx *ctx, int nImages, const int32_t *counts, const float *desc);
+
 /* Sets the pairs to match: pairImages[2p] = left image, [2p+1] = right image. */
 int v3dm_batch_set_pairs(v3dm_ctx *ctx, int nPairs, const int32_t *pairImages);
 

@@ -65,6 +65,7 @@ _SIGNATURES = {
     "v3dm_profile_enable": (ctypes.c_int, [_ctx_p, ctypes.c_int]),
     "v3dm_profile_read": (ctypes.c_int, [_ctx_p, _f64_p, _f64_p, _c_int_p]),
     "v3dm_batch_set_images_u8": (ctypes.c_int, [_ctx_p, ctypes.c_int, _i32_p, _u8_p]),
+    "v3dm_batch_set_images_f32": (ctypes.c_int, [_ctx_p, ctypes.c_int, _i32_p, _f32_p]),
     "v3dm_batch_set_pairs": (ctypes.c_int, [_ctx_p, ctypes.c_int, _i32_p]),
     "v3dm_batch_match": (ctypes.c_int, [_ctx_p, ctypes.c_float, ctypes.c_float, _i32_p, _i64_p]),
     "v3dm_batch_match_async": (ctypes.c_int, [_ctx_p, ctypes.c_float, ctypes.c_float]),

@@ -86,7 +86,7 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(2)))
    uint32_t nextReq = 0;
    if (a.persistent && tid == 0) nextReq = atomicAdd(a.tileCounter, 1u) + gridDim.x;
    PairDesc pd = a.pair0;
-   if (a.nPairs > 1) pd = a.pairs[sgpr(find_pair_by_tile(a.pairs, a.nPairs, tile))];
+   if (a.nPairs > 1) pd = a.pairs[sgpr(pair_of_tile(a.tilePair, a.pairs, a.nPairs, tile))];
    pd.offL = ((int64_t)sgpr((int)(pd.offL >> 32)) << 32) | (uint32_t)sgpr((int)pd.offL);
    pd.offR = ((int64_t)sgpr((int)(pd.offR >> 32)) << 32) | (uint32_t)sgpr((int)pd.offR);
    pd.nL = sgpr(pd.nL);
@@ -284,7 +284,7 @@ __global__ __launch_bounds__(256) void k_f32_rescore(ScanArgsF a)
    const int cnt = a.survCount[r];
    if (cnt == 0) return;
    const int tile = r / a.waves, wave = r - tile * a.waves;
-   const int p = find_pair_by_tile(a.pairs, a.nPairs, tile);
+   const int p = pair_of_tile(a.tilePair, a.pairs, a.nPairs, tile);
    const PairDesc pd = a.pairs[p];
    const int t = tile - pd.tileBase;
    const int tm = t / pd.tilesN, tn = t - tm * pd.tilesN;
@@ -313,7 +313,7 @@ __device__ inline float keyf_value(uint64_t k, uint32_t epoch)
 __global__ __launch_bounds__(256) void k_f32_finalize(FinalizeArgs a)
 {
    const int blk = blockIdx.x;
-   const int p = find_pair_by_fin(a.pairs, a.nPairs, blk);
+   const int p = pair_of_fin(a.finPair, a.pairs, a.nPairs, blk);
    const PairDesc pd = a.pairs[p];
    const int k = (blk - pd.finBase) * 256 + threadIdx.x;
    const uint32_t epoch = *a.epoch;
@@ -338,6 +338,7 @@ __global__ __launch_bounds__(256) void k_f32_finalize(FinalizeArgs a)
       }
       a.matchOfLeft[pd.slotL + k] = out;
    }
+   if (!a.done) return; // the job's epoch is advanced by k_advance_epoch instead
    __syncthreads();
    if (threadIdx.x == 0)
    {
@@ -447,10 +448,11 @@ hipError_t launch_f32_finalize(const FinalizeArgs &a, hipStream_t st)
    return hipGetLastError();
 }
 
-hipError_t launch_prep_bf16(const float *src, int n, int nPad, uint16_t *dst, uint32_t *norm2Max, hipStream_t st)
+hipError_t launch_prep_bf16(const float *src, int n, int nPad, uint16_t *dst, uint32_t *norm2Max, hipStream_t st,
+                            bool resetNorm)
 {
    if (nPad <= 0) return hipSuccess;
-   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, norm2Max, 1, 0u);
+   if (resetNorm) hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, norm2Max, 1, 0u);
    hipLaunchKernelGGL(k_prep_bf16, dim3((nPad + 15) / 16), dim3(256), 0, st, src, n, nPad, dst, norm2Max);
    return hipGetLastError();
 }

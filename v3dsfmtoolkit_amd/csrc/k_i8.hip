@@ -140,7 +140,7 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(2)))
    uint32_t nextReq = 0;
    if (a.persistent && tid == 0) nextReq = atomicAdd(a.tileCounter, 1u) + gridDim.x;
    PairDesc pd = a.pair0; // single pair: straight from the kernel arguments
-   if (a.nPairs > 1) pd = a.pairs[sgpr(find_pair_by_tile(a.pairs, a.nPairs, tile))];
+   if (a.nPairs > 1) pd = a.pairs[sgpr(pair_of_tile(a.tilePair, a.pairs, a.nPairs, tile))];
    // tile parameters are workgroup-uniform: keep them in scalar registers
    pd.offL = ((int64_t)sgpr((int)(pd.offL >> 32)) << 32) | (uint32_t)sgpr((int)pd.offL);
    pd.offR = ((int64_t)sgpr((int)(pd.offR >> 32)) << 32) | (uint32_t)sgpr((int)pd.offR);
@@ -339,7 +339,7 @@ __global__ __launch_bounds__(256) void k_i8_insert(ScanArgs a)
    const int cnt = a.survCount[r];
    if (cnt == 0) return;
    const int tile = r / a.waves, wave = r - tile * a.waves;
-   const int p = find_pair_by_tile(a.pairs, a.nPairs, tile);
+   const int p = pair_of_tile(a.tilePair, a.pairs, a.nPairs, tile);
    const PairDesc pd = a.pairs[p];
    const int t = tile - pd.tileBase;
    const int tm = t / pd.tilesN, tn = t - tm * pd.tilesN;
@@ -377,11 +377,12 @@ __device__ inline float key_value(uint64_t k, uint32_t epoch)
 
 // One thread per left feature: column decision (ratio test on the left's
 // top-2), then the right side's decision for the chosen right feature and
-// the mutual check.  The last block to finish advances the epoch.
+// the mutual check.  With a block counter (single-pair jobs) the last block to
+// finish advances the epoch; large grids leave that to k_advance_epoch.
 __global__ __launch_bounds__(256) void k_i8_finalize(FinalizeArgs a)
 {
    const int blk = blockIdx.x;
-   const int p = find_pair_by_fin(a.pairs, a.nPairs, blk);
+   const int p = pair_of_fin(a.finPair, a.pairs, a.nPairs, blk);
    const PairDesc pd = a.pairs[p];
    const int k = (blk - pd.finBase) * 256 + threadIdx.x;
    const uint32_t epoch = *a.epoch;
@@ -406,6 +407,7 @@ __global__ __launch_bounds__(256) void k_i8_finalize(FinalizeArgs a)
       }
       a.matchOfLeft[pd.slotL + k] = out;
    }
+   if (!a.done) return; // the job's epoch is advanced by k_advance_epoch instead
    __syncthreads();
    if (threadIdx.x == 0)
    {
@@ -419,6 +421,15 @@ __global__ __launch_bounds__(256) void k_i8_finalize(FinalizeArgs a)
          __threadfence();
       }
    }
+}
+
+// Closes a job whose finalize grid does not count its blocks (large batches,
+// where thousands of blocks would serialise on one counter): the next job's
+// epoch, and the persistent scan's queue head reset.  One thread.
+__global__ void k_advance_epoch(uint32_t *epoch, uint32_t *tileCounter)
+{
+   *tileCounter = 0;
+   *epoch = *epoch + 1;
 }
 
 // Sets every key slot to 0 (an epoch no job uses) and the epoch to 1.
@@ -517,6 +528,12 @@ hipError_t launch_clear_keys(uint64_t *keys, int64_t n, uint32_t *epoch, uint32_
    int64_t blocks = (n + 255) / 256;
    if (blocks < 1) blocks = 1;
    hipLaunchKernelGGL(k_clear_keys, dim3((unsigned)blocks), dim3(256), 0, st, keys, n, epoch, done);
+   return hipGetLastError();
+}
+
+hipError_t launch_advance_epoch(uint32_t *epoch, uint32_t *tileCounter, hipStream_t st)
+{
+   hipLaunchKernelGGL(k_advance_epoch, dim3(1), dim3(1), 0, st, epoch, tileCounter);
    return hipGetLastError();
 }
 

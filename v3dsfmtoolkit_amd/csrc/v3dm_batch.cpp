@@ -32,6 +32,16 @@ struct BatchState
    int64_t sumCap = 0;  // rows of dSum
    uint8_t *dRaw = nullptr;
    int64_t rawCap = 0;
+   // float images (v3dm_batch_set_images_f32): FP32 and BF16 pools at the same row offsets
+   bool isFloat = false;
+   float *dF32 = nullptr;
+   int64_t f32Cap = 0;
+   uint16_t *dBF = nullptr;
+   int64_t bfCap = 0;
+   float *dRawF = nullptr;
+   int64_t rawFCap = 0;
+   uint32_t *dNorm = nullptr;
+   float norm2Max = 0.0f;
 
    // pairs (user order) and the tile table (largest pairs first)
    int nPairs = 0;
@@ -42,6 +52,9 @@ struct BatchState
    std::vector<PairDesc> table;
    PairDesc *dPairs = nullptr;
    int pairsCap = 0;
+   std::vector<int32_t> tilePair, finPair; // table index of every tile / finalize block
+   int32_t *dTilePair = nullptr, *dFinPair = nullptr;
+   int64_t tilePairCap = 0, finPairCap = 0;
    int variant = -1;            // tile variant of the current table (-1: none yet)
    int wantVariant = -1;        // -1: autotune / tuning cache
    int nTiles = 0, nFin = 0;
@@ -54,6 +67,7 @@ struct BatchState
    uint64_t *dSurv = nullptr;
    int32_t *dSurvCount = nullptr;
    int64_t regionCap = 0;
+   int64_t survEntries = 0;     // capacity of dSurv in entries
    uint32_t *dCounters = nullptr; // epoch, done, tile counter
    uint32_t hostEpoch = 1;
    bool keysFresh = false;
@@ -73,7 +87,13 @@ void freeBatch(v3dm_ctx *c)
    dfree(b->dI8);
    dfree(b->dSum);
    dfree(b->dRaw);
+   dfree(b->dF32);
+   dfree(b->dBF);
+   dfree(b->dRawF);
+   dfree(b->dNorm);
    dfree(b->dPairs);
+   dfree(b->dTilePair);
+   dfree(b->dFinPair);
    dfree(b->dKeys);
    dfree(b->dMatch);
    dfree(b->dSurv);
@@ -108,6 +128,8 @@ void buildTable(BatchState *b, int v)
 {
    const int tm = scan_variant_tile_m(v), tn = scan_variant_tile_n(v);
    b->table.assign(b->nPairs, PairDesc{});
+   b->tilePair.clear();
+   b->finPair.clear();
    int tiles = 0, fin = 0;
    for (int i = 0; i < b->nPairs; ++i)
    {
@@ -127,6 +149,8 @@ void buildTable(BatchState *b, int v)
       d.finBase = fin;
       tiles += d.tilesM * d.tilesN;
       fin += std::max(1, (d.nL + 255) / 256);
+      b->tilePair.insert(b->tilePair.end(), (size_t)d.tilesM * d.tilesN, i);
+      b->finPair.insert(b->finPair.end(), (size_t)std::max(1, (d.nL + 255) / 256), i);
    }
    b->nTiles = tiles;
    b->nFin = fin;
@@ -145,11 +169,21 @@ int uploadTable(v3dm_ctx *c, BatchState *b)
    if (b->nPairs > 0)
       HIPCHK(c, hipMemcpyAsync(b->dPairs, b->table.data(), sizeof(PairDesc) * b->nPairs, hipMemcpyHostToDevice,
                                c->stream));
+   int rc0 = growDevice(c, b->dTilePair, b->tilePairCap, (int64_t)b->tilePair.size());
+   if (!rc0) rc0 = growDevice(c, b->dFinPair, b->finPairCap, (int64_t)b->finPair.size());
+   if (rc0) return rc0;
+   if (!b->tilePair.empty())
+      HIPCHK(c, hipMemcpyAsync(b->dTilePair, b->tilePair.data(), sizeof(int32_t) * b->tilePair.size(),
+                               hipMemcpyHostToDevice, c->stream));
+   if (!b->finPair.empty())
+      HIPCHK(c, hipMemcpyAsync(b->dFinPair, b->finPair.data(), sizeof(int32_t) * b->finPair.size(),
+                               hipMemcpyHostToDevice, c->stream));
    const int64_t regions = (int64_t)b->nTiles * scan_variant_waves(b->variant);
    int rc = growDevice(c, b->dSurvCount, b->regionCap, regions);
    if (rc) return rc;
-   dfree(b->dSurv);
-   HIPCHK(c, hipMalloc((void **)&b->dSurv, sizeof(uint64_t) * (size_t)std::max<int64_t>(regions, 1) * c->survCap));
+   // grown only when a table needs more: fresh allocations make the next job slow (first touch)
+   rc = growDevice(c, b->dSurv, b->survEntries, std::max<int64_t>(regions, 1) * c->survCap);
+   if (rc) return rc;
    b->graphValid = false;
    return syncStream(c);
 }
@@ -191,6 +225,7 @@ void fillArgs(v3dm_ctx *c, BatchState *b, float minScore, float minRatio, ScanAr
    a.tileM = scan_variant_tile_m(v);
    a.tileN = scan_variant_tile_n(v);
    a.tileCounter = b->dCounters + 2;
+   a.tilePair = b->dTilePair;
    const int resident = scan_variant_resident(v, c->device);
    a.persistent = b->nTiles > resident ? 1 : 0;
    grid = a.persistent ? resident : b->nTiles;
@@ -203,30 +238,80 @@ void fillArgs(v3dm_ctx *c, BatchState *b, float minScore, float minRatio, ScanAr
    f.top1R = a.top1R;
    f.top2R = a.top2R;
    f.epoch = b->dCounters;
-   f.done = b->dCounters + 1;
+   f.done = nullptr; // thousands of blocks: k_advance_epoch closes the job instead
    f.tileCounter = b->dCounters + 2;
    f.minScore = minScore;
    f.minRatio = minRatio;
    f.matchOfLeft = b->dMatch;
+   f.finPair = b->dFinPair;
+}
+
+// Float images: the BF16 candidate scan, FP32 rescoring, float-key finalize.
+void fillArgsF(v3dm_ctx *c, BatchState *b, float minScore, float minRatio, ScanArgsF &a, int &grid)
+{
+   const int v = b->variant;
+   uint64_t *k = b->dKeys;
+   a = ScanArgsF{};
+   a.pair0 = b->nPairs == 1 ? b->table[0] : PairDesc{};
+   a.L = b->dBF;
+   a.R = b->dBF;
+   a.Lf = b->dF32;
+   a.Rf = b->dF32;
+   a.pairs = b->dPairs;
+   a.nPairs = b->nPairs;
+   a.nTiles = b->nTiles;
+   a.thr = survivorThresholdF(minScore, minRatio);
+   a.lim = bf16CandidateLimit(a.thr, b->norm2Max, b->norm2Max);
+   a.top1L = k;
+   a.top2L = k + b->keyCapL;
+   a.top1R = k + 2 * b->keyCapL;
+   a.top2R = k + 2 * b->keyCapL + b->keyCapR;
+   a.epoch = b->dCounters;
+   a.surv = b->dSurv;
+   a.survCount = b->dSurvCount;
+   a.survCap = c->survCap;
+   a.waves = scan_variant_waves(v);
+   a.tileM = scan_variant_tile_m(v);
+   a.tileN = scan_variant_tile_n(v);
+   a.tileCounter = b->dCounters + 2;
+   a.tilePair = b->dTilePair;
+   const int resident = bf16_scan_resident(v, c->device);
+   a.persistent = b->nTiles > resident ? 1 : 0;
+   grid = a.persistent ? resident : b->nTiles;
 }
 
 int enqueueKernels(v3dm_ctx *c, BatchState *b, float minScore, float minRatio, bool marks)
 {
+   if (b->isFloat)
+   {
+      ScanArgs unused{};
+      FinalizeArgs f{};
+      int grid = 0;
+      fillArgs(c, b, minScore, minRatio, unused, f, grid);
+      ScanArgsF a{};
+      fillArgsF(c, b, minScore, minRatio, a, grid);
+      // batched jobs run for milliseconds: plain event records around the scan
+      int rc;
+      if (marks && (rc = profMark(c, 0))) return rc;
+      HIPCHK(c, launch_bf16_scan(b->variant, grid, a, c->stream));
+      if (marks && (rc = profMark(c, 1))) return rc;
+      HIPCHK(c, launch_f32_rescore(a, c->stream));
+      HIPCHK(c, launch_f32_finalize(f, c->stream));
+      HIPCHK(c, launch_advance_epoch(b->dCounters, b->dCounters + 2, c->stream));
+      if (marks && (rc = profMark(c, 2))) return rc;
+      return V3DM_OK;
+   }
    ScanArgs a{};
    FinalizeArgs f{};
    int grid = 0;
    fillArgs(c, b, minScore, minRatio, a, f, grid);
    int rc;
-   hipEvent_t e0 = nullptr, e1 = nullptr;
-   if (marks && (rc = profKernelEvents(c, &e0, &e1))) return rc;
-   if (a.nTiles <= 0 && e0)
-   {
-      HIPCHK(c, hipEventRecord(e0, c->stream));
-      HIPCHK(c, hipEventRecord(e1, c->stream));
-   }
-   HIPCHK(c, launch_i8_scan(b->variant, grid, a, c->stream, e0, e1));
+   if (marks && (rc = profMark(c, 0))) return rc;
+   HIPCHK(c, launch_i8_scan(b->variant, grid, a, c->stream));
+   if (marks && (rc = profMark(c, 1))) return rc;
    HIPCHK(c, launch_i8_insert(a, c->stream));
    HIPCHK(c, launch_i8_finalize(f, c->stream));
+   HIPCHK(c, launch_advance_epoch(b->dCounters, b->dCounters + 2, c->stream));
    if (marks && (rc = profMark(c, 2))) return rc;
    return V3DM_OK;
 }
@@ -250,7 +335,7 @@ std::string tuneKey(const BatchState *b)
    const int np = std::max(b->nPairs, 1);
    auto lg = [](double x) { return (int)std::lround(std::log2(std::max(x, 1.0))); };
    std::ostringstream os;
-   os << "i8 pairs=" << lg(np) << " nL=" << lg(sl / np) << " nR=" << lg(sr / np);
+   os << (b->isFloat ? "bf16" : "i8") << " pairs=" << lg(np) << " nL=" << lg(sl / np) << " nR=" << lg(sr / np);
    return os.str();
 }
 
@@ -353,7 +438,7 @@ int pickVariant(v3dm_ctx *c, BatchState *b, float minScore, float minRatio)
 
 int enqueueJob(v3dm_ctx *c, BatchState *b, float minScore, float minRatio)
 {
-   if (b->hostEpoch >= kEpochLimit)
+   if (b->hostEpoch >= (b->isFloat ? kEpochLimitF : kEpochLimit))
    {
       int rc = clearBatchKeys(c, b);
       if (rc) return rc;
@@ -453,6 +538,78 @@ int v3dm_batch_set_images_u8(v3dm_ctx *c, int nImages, const int32_t *counts, co
    }
    rc = syncStream(c);
    if (rc) return rc;
+   b->isFloat = false;
+   b->nImages = nImages;
+   b->counts = cnt;
+   b->imgOff = off;
+   b->poolRows = rows;
+   b->nPairs = 0; // pairs refer to images: set them again
+   b->variant = -1;
+   b->graphValid = false;
+   return V3DM_OK;
+}
+
+int v3dm_batch_set_images_f32(v3dm_ctx *c, int nImages, const int32_t *counts, const float *desc)
+{
+   if (!c) return V3DM_ERR_ARG;
+   if (nImages < 0 || (nImages > 0 && !counts)) return fail(c, V3DM_ERR_ARG, "batch_set_images: bad arguments");
+   HIPCHK(c, hipSetDevice(c->device));
+   BatchState *b = batchOf(c);
+   if (b->pending)
+   {
+      int rc = syncStream(c);
+      if (rc) return rc;
+      b->pending = false;
+   }
+   int64_t total = 0, rows = 0;
+   std::vector<int> cnt(nImages);
+   std::vector<int64_t> off(nImages);
+   for (int i = 0; i < nImages; ++i)
+   {
+      if (counts[i] < 0 || counts[i] > kMaxLocalIndex)
+         return fail(c, V3DM_ERR_ARG, "batch_set_images: image %d has %d features", i, counts[i]);
+      int n = counts[i];
+      if (n % 32)
+      {
+         if (c->warnRounding)
+            fprintf(stderr, "SIFT_Matching batch warning: image %d has %d features, rounding down to a multiple "
+                            "of 32\n", i, n);
+         n -= n & 31;
+      }
+      cnt[i] = n;
+      off[i] = rows;
+      rows += std::max(roundUp(std::max(n, 1), kRowPad), kRowPad);
+      total += counts[i];
+   }
+   if (total > 0 && !desc) return fail(c, V3DM_ERR_ARG, "batch_set_images: descriptor pointer is NULL");
+   int rc = growDevice(c, b->dF32, b->f32Cap, rows * kDim);
+   if (!rc) rc = growDevice(c, b->dBF, b->bfCap, rows * kDim);
+   if (!rc) rc = growDevice(c, b->dRawF, b->rawFCap, total * kDim);
+   if (rc) return rc;
+   if (!b->dNorm) HIPCHK(c, hipMalloc((void **)&b->dNorm, sizeof(uint32_t)));
+   if (total > 0)
+      HIPCHK(c, hipMemcpyAsync(b->dRawF, desc, sizeof(float) * (size_t)total * kDim, hipMemcpyHostToDevice,
+                               c->stream));
+   int64_t src = 0;
+   for (int i = 0; i < nImages; ++i)
+   {
+      const int nPad = std::max(roundUp(std::max(cnt[i], 1), kRowPad), kRowPad);
+      if (cnt[i] > 0)
+         HIPCHK(c, hipMemcpyAsync(b->dF32 + off[i] * kDim, b->dRawF + src * kDim, sizeof(float) * (size_t)cnt[i] * kDim,
+                                  hipMemcpyDeviceToDevice, c->stream));
+      HIPCHK(c, launch_prep_bf16(b->dF32 + off[i] * kDim, cnt[i], nPad, b->dBF + off[i] * kDim, b->dNorm, c->stream,
+                                 i == 0));
+      src += counts[i];
+   }
+   uint32_t bits = 0;
+   if (nImages > 0)
+      HIPCHK(c, hipMemcpyAsync(c->hFlag ? (void *)c->hFlag : (void *)&bits, b->dNorm, sizeof(uint32_t),
+                               hipMemcpyDeviceToHost, c->stream));
+   rc = syncStream(c);
+   if (rc) return rc;
+   if (nImages > 0 && c->hFlag) memcpy(&bits, c->hFlag, sizeof(bits));
+   memcpy(&b->norm2Max, &bits, sizeof(bits));
+   b->isFloat = true;
    b->nImages = nImages;
    b->counts = cnt;
    b->imgOff = off;

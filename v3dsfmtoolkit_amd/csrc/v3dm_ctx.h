@@ -27,8 +27,10 @@ hipError_t launch_bf16_scan(int variant, int grid, const ScanArgsF &a, hipStream
 int bf16_scan_resident(int variant, int device);
 hipError_t launch_f32_rescore(const ScanArgsF &a, hipStream_t st);
 hipError_t launch_f32_finalize(const FinalizeArgs &a, hipStream_t st);
-hipError_t launch_prep_bf16(const float *src, int n, int nPad, uint16_t *dst, uint32_t *norm2Max, hipStream_t st);
+hipError_t launch_prep_bf16(const float *src, int n, int nPad, uint16_t *dst, uint32_t *norm2Max, hipStream_t st,
+                            bool resetNorm = true);
 hipError_t launch_i8_insert(const ScanArgs &a, hipStream_t st);
+hipError_t launch_advance_epoch(uint32_t *epoch, uint32_t *tileCounter, hipStream_t st);
 hipError_t launch_clear_keys(uint64_t *keys, int64_t n, uint32_t *epoch, uint32_t *done, hipStream_t st);
 hipError_t launch_mfma_probe(int32_t *out, hipStream_t st);
 hipError_t launch_copy_probe(const void *src, void *dst, int64_t bytes, hipStream_t st);

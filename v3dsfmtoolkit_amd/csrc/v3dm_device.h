@@ -36,6 +36,18 @@ __device__ inline int find_pair_by_fin(const PairDesc *pairs, int nPairs, int bl
    return lo;
 }
 
+// Pair of a tile / finalize block: one load from the precomputed table when
+// the batch provides it, else a binary search over the pairs' first indices.
+__device__ inline int pair_of_tile(const int32_t *tilePair, const PairDesc *pairs, int nPairs, int tile)
+{
+   return tilePair ? tilePair[tile] : find_pair_by_tile(pairs, nPairs, tile);
+}
+
+__device__ inline int pair_of_fin(const int32_t *finPair, const PairDesc *pairs, int nPairs, int blk)
+{
+   return finPair ? finPair[blk] : find_pair_by_fin(pairs, nPairs, blk);
+}
+
 // Epipolar Sampson test, v3d_cudamatching.cu:266-290 (the kernels are
 // compiled with -ffp-contract=off, so the expression is evaluated as written).
 __device__ inline bool mask_hit(const Mask &mk, float px, float py, float qx, float qy)

@@ -92,6 +92,7 @@ struct ScanArgs
    int persistent;              // 1: resident workgroups pull tiles from *tileCounter
    uint32_t *tileCounter;       // reset to 0 by the finalize kernel
    const int32_t *mapL, *mapR;  // optional: feature index of each row of a gathered pool
+   const int32_t *tilePair;     // optional: index into pairs of every tile (else a binary search)
 };
 
 // Float descriptors: BF16 MFMA candidate scan + exact FP32 rescoring.
@@ -115,6 +116,7 @@ struct ScanArgsF
    int waves, tileM, tileN;
    int persistent;
    uint32_t *tileCounter;
+   const int32_t *tilePair;  // optional: index into pairs of every tile
 };
 
 struct FinalizeArgs
@@ -124,10 +126,11 @@ struct FinalizeArgs
    int nBlocks;
    const uint64_t *top1L, *top2L, *top1R, *top2R;
    uint32_t *epoch;
-   uint32_t *done;
+   uint32_t *done;              // block counter of the job's last-block epoch advance; null: none
    uint32_t *tileCounter;       // persistent scan's queue head, reset at the end of the job
    float minScore, minRatio;
    int32_t *matchOfLeft; // per left slot: matched right index or -1
+   const int32_t *finPair; // optional: index into pairs of every finalize block
 };
 
 } // namespace v3dm

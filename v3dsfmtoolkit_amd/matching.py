@@ -163,13 +163,19 @@ class SIFTMatching:
 
     # -- many pairs per job (v3dm_batch_*) ---------------------------------------
     def setImages(self, images) -> list[int]:
-        """Uploads uint8 descriptor sets (list of (n_i, 128) arrays); returns
-        the counts after rounding to multiples of 32."""
+        """Uploads descriptor sets (list of (n_i, 128) arrays, all uint8 or all
+        float32); returns the counts after rounding to multiples of 32."""
         counts = np.array([len(x) for x in images], np.int32)
-        desc = np.ascontiguousarray(np.concatenate([np.asarray(x, np.uint8).reshape(-1, 128) for x in images])
-                                    if len(images) else np.zeros((0, 128), np.uint8))
-        check(self._lib.v3dm_batch_set_images_u8(self._ctx, len(images), ptr(counts, ctypes.c_int32),
-                                                 ptr(desc, ctypes.c_uint8)), self._ctx)
+        is_float = len(images) > 0 and np.asarray(images[0]).dtype != np.uint8
+        dt = np.float32 if is_float else np.uint8
+        desc = np.ascontiguousarray(np.concatenate([np.asarray(x, dt).reshape(-1, 128) for x in images])
+                                    if len(images) else np.zeros((0, 128), dt))
+        if is_float:
+            check(self._lib.v3dm_batch_set_images_f32(self._ctx, len(images), ptr(counts, ctypes.c_int32),
+                                                      ptr(desc, ctypes.c_float)), self._ctx)
+        else:
+            check(self._lib.v3dm_batch_set_images_u8(self._ctx, len(images), ptr(counts, ctypes.c_int32),
+                                                     ptr(desc, ctypes.c_uint8)), self._ctx)
         self._img_counts = [int(c) - int(c) % 32 for c in counts]
         return self._img_counts
 
