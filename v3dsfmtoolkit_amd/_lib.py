@@ -14,7 +14,9 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libv3dmatch.so")
-HEADER_PATH = os.path.join(os.path.dirname(HERE), "include", "v3dmatch.h")
+INCLUDE_DIR = os.path.join(os.path.dirname(HERE), "include")
+HEADER_PATH = os.path.join(INCLUDE_DIR, "v3dmatch.h")
+HEADER_PATHS = [os.path.join(INCLUDE_DIR, h) for h in ("v3dmatch.h", "v3dm_io.h")]
 
 V3DM_OK = 0
 V3DM_ERR_ARG = -1
@@ -101,16 +103,37 @@ _SIGNATURES = {
     "v3dm_learned_find": (ctypes.c_int, [_ctx_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_float,
                                          ctypes.c_float, _i32_p, ctypes.c_int, _c_int_p, _i64_p]),
     "v3dm_learned_candidates": (ctypes.c_int, [_ctx_p, ctypes.c_int, _i32_p, _c_int_p]),
+    # include/v3dm_io.h (host only)
+    "v3dm_features_info": (ctypes.c_int, [ctypes.c_char_p, _c_int_p, _c_int_p]),
+    "v3dm_features_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, _i32_p, _f32_p, _f32_p, _f32_p, _f32_p,
+                                          _f32_p, _c_int_p]),
+    "v3dm_features_write": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, _i32_p, _f32_p, _f32_p, _f32_p, _f32_p,
+                                           _f32_p, ctypes.c_int]),
+    "v3dm_truncated_feature_count": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int]),
+    "v3dm_sequence_pairs": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _i32_p, ctypes.c_int, _c_int_p]),
+    "v3dm_descriptors_to_u8": (ctypes.c_int, [ctypes.c_int64, _f32_p, _u8_p]),
+    "v3dm_pairwise_match_size": (ctypes.c_size_t, [ctypes.c_int]),
+    "v3dm_encode_pairwise_match": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _f64_p, _f64_p, ctypes.c_int,
+                                                  ctypes.c_void_p, _u8_p, ctypes.c_size_t,
+                                                  ctypes.POINTER(ctypes.c_size_t)]),
+    "v3dm_decode_pairwise_match": (ctypes.c_int, [_u8_p, ctypes.c_size_t, _c_int_p, _c_int_p, _f64_p, _f64_p,
+                                                  ctypes.c_void_p, ctypes.c_int, _c_int_p]),
+    "v3dm_match_db_open": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    "v3dm_match_db_put": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f64_p, _f64_p,
+                                         ctypes.c_int, ctypes.c_void_p]),
+    "v3dm_match_db_close": (ctypes.c_int, [ctypes.c_void_p]),
 }
 
 _lib = None
 
 
-def header_symbols(path: str = HEADER_PATH) -> list[str]:
-    """Names of every function declared in include/v3dmatch.h."""
-    text = open(path).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(v3dm_[a-z0-9_]+)\s*\(", text)))
+def header_symbols(paths=None) -> list[str]:
+    """Names of every function declared in include/v3dmatch.h and include/v3dm_io.h."""
+    names = set()
+    for path in paths or HEADER_PATHS:
+        text = re.sub(r"/\*.*?\*/", "", open(path).read(), flags=re.S)
+        names.update(re.findall(r"\b(v3dm_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)
 
 
 def lib() -> ctypes.CDLL:
