@@ -46,6 +46,8 @@ _SIG = {
     "orc_find_u8_blocked": (ctypes.c_int, [ctypes.c_int, _u8, ctypes.c_int, _u8, ctypes.c_float, ctypes.c_float,
                                            ctypes.c_int, _i32, _i32, _i32]),
     "orc_max_threads": (ctypes.c_int, []),
+    "orc_voctree_word": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _f32, _f32]),
+    "orc_voctree_quantize": (None, [ctypes.c_int, ctypes.c_int, _f32, ctypes.c_int64, _f32, _i32]),
 }
 
 _lib = None

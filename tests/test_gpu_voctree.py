@@ -1,0 +1,150 @@
+"""GPU parity of vocabulary-tree quantisation (v3dm_voctree_*): the words must
+equal the CPU oracle's restatement of VocabularyTree::findBestLeafIndex
+(V3D/include/Base/v3d_voctree.h:63-99) bit for bit -- on the driver's tree
+shape (depth 3, branch factor 50, match_using_voctree.cpp:145), on other
+shapes, and on the crafted cases that separate the sequential unfused float
+sum from other summations."""
+import sys
+
+import numpy as np
+import pytest
+
+import v3dsfmtoolkit_amd as v
+from oracle import voctree as OV
+from v3dsfmtoolkit_amd import voctree as VT
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def kmeans_like_tree(seed, depth, bf, spread=0.35):
+    """Unit-norm node descriptors, each child a perturbation of its parent
+    (what hierarchical k-means on SIFT produces, roughly)."""
+    rng = np.random.default_rng(seed)
+    n = VT.tree_nodes(depth, bf)
+    tree = np.zeros((n, 128), np.float32)
+    start, ln = 0, bf
+    parents = [np.abs(rng.normal(size=128))]
+    for d in range(depth):
+        level = []
+        for p in parents:
+            c = np.abs(p + spread * rng.normal(size=(bf, 128)))
+            c /= np.linalg.norm(c, axis=1, keepdims=True)
+            level.extend(c)
+        tree[start:start + ln] = np.asarray(level, np.float32)
+        parents = level
+        start += ln
+        ln *= bf
+    return tree
+
+
+def keys_like_sift(seed, n):
+    d = v.synth_pair(v.synth_seed(6, seed), n, 0, u8=False, f32=True)["Lf"]
+    return np.ascontiguousarray(d, np.float32)
+
+
+@pytest.fixture(scope="module")
+def big_tree(gpu_available):
+    say("building the 3 x 50 tree")
+    tree = kmeans_like_tree(1, 3, 50)
+    t = VT.VocabularyTree(3, 50, tree)
+    yield tree, t
+    t.close()
+
+
+def test_driver_tree_shape_matches_oracle(big_tree):
+    tree, t = big_tree
+    assert t.visualWordCount() == 125000
+    keys = keys_like_sift(1, 20000)
+    say("quantise 20000 unit keys")
+    got = t.getVisualWords(keys)
+    want = OV.quantise(3, 50, tree, keys)
+    np.testing.assert_array_equal(got, want)
+    assert len(np.unique(want)) > 2000  # the keys spread over the tree
+    q = np.clip(np.rint(keys * 512), 0, 255).astype(np.float32) / 512  # u/512 keys
+    np.testing.assert_array_equal(t.getVisualWords(q), OV.quantise(3, 50, tree, q))
+    scoring, run = t.runTimes()
+    assert 0 < scoring <= run
+
+
+def test_repeated_runs_and_smaller_jobs(big_tree):
+    tree, t = big_tree
+    for n in (5000, 1, 65, 0, 777):
+        say("job of", n, "keys")
+        keys = keys_like_sift(100 + n, max(n, 1))[:n]
+        t.upload(keys)
+        t.run()
+        t.run()
+        np.testing.assert_array_equal(t.fetch(), OV.quantise(3, 50, tree, keys))
+
+
+@pytest.mark.parametrize("depth,bf", [(1, 7), (2, 300), (5, 10), (4, 13)])
+def test_other_tree_shapes(gpu_available, depth, bf):
+    say("tree shape", depth, bf)
+    tree = kmeans_like_tree(depth * 1000 + bf, depth, bf)
+    t = VT.VocabularyTree(depth, bf, tree)
+    try:
+        keys = keys_like_sift(depth * 7 + bf, 3000)
+        np.testing.assert_array_equal(t.getVisualWords(keys), OV.quantise(depth, bf, tree, keys))
+    finally:
+        t.close()
+
+
+def test_crafted_rounding_and_tie_cases(gpu_available):
+    """Each key's answer differs between the reference's unfused sequential float
+    sum and (a) the exact sum, (b) a fused multiply-add order; plus exact ties
+    (first child wins) and a non-finite key (-1)."""
+    depth, bf = 2, 4
+    tree = np.zeros((VT.tree_nodes(depth, bf), 128), np.float32)
+    keys = []
+    # root child 0 vs 1: cancellation -- exact score of child 0 is 1, float sum 0; child 1 scores 0.5
+    tree[0, :3] = 1.0
+    tree[1, 3] = 0.5
+    k = np.zeros(128, np.float32)
+    k[:4] = [1e8, 1.0, -1e8, 1.0]
+    keys.append(k)
+    # root child 2 vs 3 for a key orthogonal to children 0/1: fused vs unfused products
+    x = np.float32(1 + 2.0 ** -12)
+    tree[2, 10], tree[2, 11] = -1.0, x
+    tree[3, 12] = np.float32(2.0 ** -11 + 2.0 ** -25)
+    k = np.zeros(128, np.float32)
+    k[10], k[11], k[12] = 1.0, x, 1.0
+    keys.append(k)
+    # second level below child 1 (nodes 8..11): two identical children 9 and 10 -> 9 (index 1)
+    tree[9, 20] = tree[10, 20] = 0.25
+    k = np.zeros(128, np.float32)
+    k[3], k[20] = 1.0, 1.0
+    keys.append(k)
+    k = keys[0].copy()
+    k[50] = np.nan
+    keys.append(k)
+    keys = np.asarray(keys, np.float32)
+    want = OV.quantise(depth, bf, tree, keys)
+    assert want[0] // bf == 1 and want[1] // bf == 3 and want[2] == 1 * bf + 1 and want[3] == -1
+    t = VT.VocabularyTree(depth, bf, tree)
+    try:
+        np.testing.assert_array_equal(t.getVisualWords(keys), want)
+        many = np.repeat(keys, 97, axis=0)  # the same cases spread over many waves and groups
+        np.testing.assert_array_equal(t.getVisualWords(many), np.repeat(want, 97))
+    finally:
+        t.close()
+
+
+def test_tree_file_round_trip(gpu_available, tmp_path):
+    depth, bf = 2, 9
+    tree = kmeans_like_tree(5, depth, bf)
+    p = tmp_path / "tree.bin"
+    tree.tofile(p)
+    keys = keys_like_sift(9, 500)
+    t = VT.VocabularyTree(depth, bf, path=p)
+    try:
+        np.testing.assert_array_equal(t.getVisualWords(keys), OV.quantise(depth, bf, tree, keys))
+    finally:
+        t.close()
+    p.write_bytes(tree.tobytes()[:-4])
+    with pytest.raises(v.V3DMError, match="needs"):
+        VT.VocabularyTree(depth, bf, path=p)

@@ -8,8 +8,8 @@ driver bookkeeping of include/v3dm_io.h.
 from ._lib import (KERNEL_AUTO, KERNEL_BF16, KERNEL_I8, KERNEL_PLAIN, V3DMError, copy_probe, device_count, lib,
                    mfma_layout_probe, scan_variants, synth_pair, synth_seed)
 from .matching import EmbeddingNet, SIFTMatching
-from . import drivers, features, storage  # noqa: E402  (file formats and drivers' bookkeeping, host only)
+from . import drivers, features, storage, voctree  # noqa: E402  (file formats, drivers, vocabulary tree)
 
 __all__ = ["SIFTMatching", "EmbeddingNet", "V3DMError", "KERNEL_AUTO", "KERNEL_PLAIN", "KERNEL_I8", "KERNEL_BF16", "device_count", "lib",
            "mfma_layout_probe", "copy_probe", "scan_variants", "synth_pair", "synth_seed", "drivers", "features",
-           "storage"]
+           "storage", "voctree"]

@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libv3dmatch.so")
 INCLUDE_DIR = os.path.join(os.path.dirname(HERE), "include")
 HEADER_PATH = os.path.join(INCLUDE_DIR, "v3dmatch.h")
-HEADER_PATHS = [os.path.join(INCLUDE_DIR, h) for h in ("v3dmatch.h", "v3dm_io.h")]
+HEADER_PATHS = [os.path.join(INCLUDE_DIR, h) for h in ("v3dmatch.h", "v3dm_io.h", "v3dm_voctree.h")]
 
 V3DM_OK = 0
 V3DM_ERR_ARG = -1
@@ -122,6 +122,29 @@ _SIGNATURES = {
     "v3dm_match_db_put": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f64_p, _f64_p,
                                          ctypes.c_int, ctypes.c_void_p]),
     "v3dm_match_db_close": (ctypes.c_int, [ctypes.c_void_p]),
+    # include/v3dm_voctree.h
+    "v3dm_voctree_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32_p,
+                                           ctypes.POINTER(ctypes.c_void_p)]),
+    "v3dm_voctree_load": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p,
+                                         ctypes.POINTER(ctypes.c_void_p)]),
+    "v3dm_voctree_destroy": (None, [ctypes.c_void_p]),
+    "v3dm_voctree_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
+    "v3dm_voctree_info": (ctypes.c_int, [ctypes.c_void_p, _c_int_p, _c_int_p, _c_int_p, _c_int_p]),
+    "v3dm_voctree_quantize": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, _f32_p, _i32_p]),
+    "v3dm_voctree_upload": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, _f32_p]),
+    "v3dm_voctree_run": (ctypes.c_int, [ctypes.c_void_p]),
+    "v3dm_voctree_fetch": (ctypes.c_int, [ctypes.c_void_p, _i32_p]),
+    "v3dm_voctree_run_times": (ctypes.c_int, [ctypes.c_void_p, _f64_p, _f64_p]),
+    "v3dm_leaf_documents_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    "v3dm_leaf_documents_destroy": (None, [ctypes.c_void_p]),
+    "v3dm_leaf_documents_insert": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _i32_p, _c_int_p]),
+    "v3dm_leaf_documents_count": (ctypes.c_int, [ctypes.c_void_p, _c_int_p]),
+    "v3dm_leaf_documents_binomial_scores": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, _i32_p,
+                                                           ctypes.c_int, _f32_p, _i32_p, ctypes.c_int, _c_int_p]),
+    "v3dm_leaf_documents_l1_scores": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _i32_p, ctypes.c_int, _f32_p,
+                                                     _i32_p, ctypes.c_int, _c_int_p]),
+    "v3dm_voctree_candidate_pairs": (ctypes.c_int, [ctypes.c_int, _i32_p, _i32_p, ctypes.c_int, ctypes.c_int,
+                                                    ctypes.c_double, _i32_p, _f32_p, ctypes.c_int, _c_int_p]),
 }
 
 _lib = None
