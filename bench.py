@@ -84,9 +84,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run a plain device copy of this size (scales memory counters)")
-    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5"], default="c2",
+    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5", "voctree"], default="c2",
                     help="c2: single pair 8192^2 (default line); c3: 256 uneven pairs in one batch (+ float "
-                         "variant); c4: learned path on 64 pairs of 16384^2; c5: single pair 65536^2")
+                         "variant); c4: learned path on 64 pairs of 16384^2; c5: single pair 65536^2; voctree: "
+                         "vocabulary-tree quantisation of 64 views x 7168 features (depth 3, branch factor 50)")
     ap.add_argument("--pairs", type=int, default=0, help="number of pairs for c3 / c4 (0: the workload's own)")
     ap.add_argument("--scan-variant", type=int, default=-1,
                     help="single-pair int8 scan variant (-1: the library measures the fastest)")
@@ -102,7 +103,7 @@ def main():
     import v3dsfmtoolkit_amd as v
 
     if args.workload != "c2":
-        out = {"c3": run_c3, "c4": run_c4, "c5": run_c5}[args.workload](args, v)
+        out = {"c3": run_c3, "c4": run_c4, "c5": run_c5, "voctree": run_voctree}[args.workload](args, v)
         print(json.dumps(out), flush=True)
         return
     run_c2(args, v)
@@ -520,6 +521,66 @@ def run_c5(args, v):
     out["plain"] = {"ms_per_pair": dt_p * 1e3, "equals_int8": plain_same}
     out["cpu_baseline"] = {"value": cpu, "unit": "descriptor pairs scored/s", "cores": threads, "kind": "port",
                            "sample": "2048 left rows against all 65536 rights, oracle blocked rule"}
+    return out
+
+
+def run_voctree(args, v):
+    """Vocabulary-tree quantisation (the voctree driver's first stage) of 64
+    views x 7168 SIFT-like unit float descriptors on a depth-3, branch-factor-50
+    tree, keys resident in HBM.  A step is one full quantisation of all keys:
+    per level the regrouping kernels and the scoring kernel."""
+    from oracle import voctree as OV
+    from v3dsfmtoolkit_amd import voctree as VT
+    depth, bf, per = 3, 50, 7168
+    views = args.pairs or 64
+    tree = VT.synthetic_tree(1, depth, bf)
+    keys = np.concatenate([v.synth_pair(v.synth_seed(7, i), per, 0, u8=False, f32=True)["Lf"] for i in range(views)])
+    keys = np.ascontiguousarray(keys, np.float32)
+    n = len(keys)
+    t = VT.VocabularyTree(depth, bf, tree)
+    t.upload(keys)
+    steps, warmup = min(args.steps, 200), min(args.warmup, 20)
+    for _ in range(max(warmup, 1)):
+        t.run()
+    words = t.fetch()
+    sample = min(n, 50000)
+    same = bool(np.array_equal(words[:sample], OV.quantise(depth, bf, tree, keys[:sample])))
+    progress(f"{n} keys quantised; first {sample} equal the oracle: {same}")
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        t.run()
+    t.runTimes()  # waits for the stream
+    wall = (time.perf_counter() - t0) / steps
+    scoring, runs = [], []
+    for _ in range(20):
+        t.run()
+        a, b = t.runTimes()
+        scoring.append(a)
+        runs.append(b)
+    t.close()
+    level_avg_s = statistics.mean(scoring) / 1e3 / depth
+    ops_per_launch = n * bf * 128 * 2  # one multiply and one add per term, per level
+    achieved = ops_per_launch / level_avg_s / 1e12
+    threads = len(os.sched_getaffinity(0))
+    cpu_n = min(n, 100000)
+    tc = time.perf_counter()
+    OV.quantise(depth, bf, tree, keys[:cpu_n])
+    cpu = cpu_n / (time.perf_counter() - tc)
+    out = _common("visual words/s", n / wall, "features quantised/s", steps, warmup, wall, "f32",
+                  {"workload": f"{views} views x {per} SIFT-like unit float descriptors, vocabulary tree depth "
+                               f"{depth}, branch factor {bf} (synthetic unit-norm nodes)", "features": n,
+                   "tree_nodes": VT.tree_nodes(depth, bf)})
+    out["roofline"] = {"bound": "valu", "kernel": "k_vt_level", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
+                       "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                       "launch_avg_us": level_avg_s * 1e6,
+                       "attainable_peak": FP32_PEAK_TFLOPS / 4,
+                       "note": "the reference's unfused sequential sums allow neither FMA nor packed FP32 "
+                               "(one v_mul_f32 + one v_add_f32 per term): 1/4 of the FP32 vector peak is "
+                               "attainable"}
+    out["run_ms"] = statistics.mean(runs)
+    out["equals_oracle_sample"] = same
+    out["cpu_baseline"] = {"value": cpu, "unit": "features quantised/s", "cores": threads, "kind": "port",
+                           "sample": f"the first {cpu_n} keys, oracle sequential descent (OpenMP over keys)"}
     return out
 
 

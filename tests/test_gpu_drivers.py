@@ -173,3 +173,44 @@ def test_python_match_views_equals_oracle(tmp_path, gpu_available):
         want = O.table_to_pairs(tab)
         assert [(int(x) - 1000 * a, int(y) - 1000 * b) for x, y in zip(c["left_id"], c["right_id"])] == want
         assert (c["left_view"] == a).all() and (c["right_view"] == b).all()
+
+
+VOCTREE_APP = os.path.join(ROOT, "v3dsfmtoolkit_amd", "bin", "v3dm_match_using_voctree")
+
+
+def test_match_using_voctree_matches_oracles(tmp_path, gpu_available):
+    """Stage 1 (quantisation + binomial candidate choice) against the voctree
+    oracles, the candidate graph file, and stage 2 against the per-pair oracle."""
+    from oracle import voctree as OV
+    from v3dsfmtoolkit_amd import voctree as VT
+    depth, bf = 2, 40
+    tree = VT.synthetic_tree(8, depth, bf)
+    tree.tofile(tmp_path / "tree.bin")
+    views = make_views(41, [900, 1100, 700, 1300, 800, 1000, 1200, 600], True)
+    conf = {"MATCHING_MAX_FEATURES": 1024, "MATCHING_SCORE_RATIO": 0.8, "MINIMUM_RAW_CORRESPONDENCES": 10,
+            "POTENTIAL_VIEWS_COUNT": 2, "VISUAL_WORD_MATCH_PROPABILITY": 0.1}
+    names = write_run(tmp_path, views, [True] * len(views), 5, conf)
+    say("v3dm_match_using_voctree")
+    r = subprocess.run([VOCTREE_APP, "--db", str(tmp_path / "out.db"), "--tree-shape", str(depth), str(bf), "tree.bin",
+                        "list.txt", "run.conf"], capture_output=True, text=True, cwd=tmp_path, timeout=100)
+    assert r.returncode == 0, r.stderr
+    fs = [F.read_features(tmp_path / f"{nm}.features") for nm in names]
+    words = OV.quantise(depth, bf, tree, np.concatenate([f.descriptors for f in fs]))
+    cands = OV.candidate_pairs([len(f) for f in fs], words, bf ** depth, 2, float(np.float32(0.1)))
+    pairs = [(a, b) for a, b, _ in cands]
+    assert len(pairs) == 2 * (len(views) - 1) - 1  # view 1 has one earlier view, the others two
+    dot = (tmp_path / "putative_EG.dot").read_text()
+    assert dot == "graph {\n" + "".join(f"{a} -- {b};\n" for a, b in pairs) + "}\n"
+    printed, cur = {}, None
+    for line in r.stdout.splitlines():
+        if line.startswith("Potential matches for "):
+            cur = line[len("Potential matches for "):-1]
+            printed[cur] = []
+        elif line.startswith(" ") and " with score " in line:
+            printed[cur].append(line.split()[0])
+    for i, nm in enumerate(names):
+        mine = sorted((s, a) for a, b, s in cands if b == i)
+        assert printed[nm] == [names[a] for _, a in mine], nm
+    want, _ = expected_rows(tmp_path, names, pairs, 1024, 0.8, 10, True)
+    assert len(want) >= 3
+    check_rows(tmp_path, want)

@@ -21,25 +21,8 @@ def say(*a):
     sys.stdout.flush()
 
 
-def kmeans_like_tree(seed, depth, bf, spread=0.35):
-    """Unit-norm node descriptors, each child a perturbation of its parent
-    (what hierarchical k-means on SIFT produces, roughly)."""
-    rng = np.random.default_rng(seed)
-    n = VT.tree_nodes(depth, bf)
-    tree = np.zeros((n, 128), np.float32)
-    start, ln = 0, bf
-    parents = [np.abs(rng.normal(size=128))]
-    for d in range(depth):
-        level = []
-        for p in parents:
-            c = np.abs(p + spread * rng.normal(size=(bf, 128)))
-            c /= np.linalg.norm(c, axis=1, keepdims=True)
-            level.extend(c)
-        tree[start:start + ln] = np.asarray(level, np.float32)
-        parents = level
-        start += ln
-        ln *= bf
-    return tree
+def kmeans_like_tree(seed, depth, bf):
+    return VT.synthetic_tree(seed, depth, bf)
 
 
 def keys_like_sift(seed, n):

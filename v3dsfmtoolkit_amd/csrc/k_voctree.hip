@@ -105,7 +105,9 @@ __global__ __launch_bounds__(kVtWaves * 64) void k_vt_level(const float *__restr
    leaf[f] = out;
 }
 
-// Group sizes of the next level: node id leaf[f] - parentBase for live features.
+// Group sizes of the next level: node id leaf[f] - parentBase for live features
+// (direct global atomics: used when the groups outnumber the LDS bins below,
+// where contention per group is low anyway).
 __global__ void k_vt_count(int n, const int32_t *__restrict__ leaf, int parentBase, int nGroups,
                            int32_t *__restrict__ counts)
 {
@@ -176,6 +178,71 @@ __global__ void k_vt_scatter(int n, const int32_t *__restrict__ leaf, int parent
    if (g >= 0 && g < nGroups) order[atomicAdd(&cursor[g], 1)] = f;
 }
 
+// The same two steps with the histogram privatised in LDS: a workgroup takes
+// kVtChunk consecutive features, counts them per group in LDS and adds each
+// non-empty bin to the global count once (one global atomic per non-empty
+// group per workgroup instead of one per feature: the root level has only
+// B groups).  The scatter reserves each bin's range with one global atomic
+// and places every feature at its LDS rank inside it.
+constexpr int kVtChunkThreads = 256, kVtPerThread = 8, kVtChunk = kVtChunkThreads * kVtPerThread;
+constexpr int kVtLdsBins = 8192;
+
+__global__ __launch_bounds__(kVtChunkThreads) void k_vt_count_lds(int n, const int32_t *__restrict__ leaf,
+                                                                 int parentBase, int nGroups,
+                                                                 int32_t *__restrict__ counts)
+{
+   __shared__ int32_t bins[kVtLdsBins];
+   for (int b = threadIdx.x; b < nGroups; b += kVtChunkThreads) bins[b] = 0;
+   __syncthreads();
+   const int f0 = blockIdx.x * kVtChunk;
+   for (int j = 0; j < kVtPerThread; ++j)
+   {
+      const int f = f0 + j * kVtChunkThreads + (int)threadIdx.x;
+      if (f < n)
+      {
+         const int g = leaf[f] - parentBase;
+         if (g >= 0 && g < nGroups) atomicAdd(&bins[g], 1);
+      }
+   }
+   __syncthreads();
+   for (int b = threadIdx.x; b < nGroups; b += kVtChunkThreads)
+      if (bins[b]) atomicAdd(&counts[b], bins[b]);
+}
+
+__global__ __launch_bounds__(kVtChunkThreads) void k_vt_scatter_lds(int n, const int32_t *__restrict__ leaf,
+                                                                   int parentBase, int nGroups,
+                                                                   int32_t *__restrict__ cursor,
+                                                                   int32_t *__restrict__ order)
+{
+   __shared__ int32_t bins[kVtLdsBins];
+   for (int b = threadIdx.x; b < nGroups; b += kVtChunkThreads) bins[b] = 0;
+   __syncthreads();
+   const int f0 = blockIdx.x * kVtChunk;
+   int g[kVtPerThread], rank[kVtPerThread];
+#pragma unroll
+   for (int j = 0; j < kVtPerThread; ++j)
+   {
+      const int f = f0 + j * kVtChunkThreads + (int)threadIdx.x;
+      g[j] = -1;
+      if (f < n)
+      {
+         const int gg = leaf[f] - parentBase;
+         if (gg >= 0 && gg < nGroups)
+         {
+            g[j] = gg;
+            rank[j] = atomicAdd(&bins[gg], 1);
+         }
+      }
+   }
+   __syncthreads();
+   for (int b = threadIdx.x; b < nGroups; b += kVtChunkThreads)
+      if (bins[b]) bins[b] = atomicAdd(&cursor[b], bins[b]); // the bin's first slot
+   __syncthreads();
+#pragma unroll
+   for (int j = 0; j < kVtPerThread; ++j)
+      if (g[j] >= 0) order[bins[g[j]] + rank[j]] = f0 + j * kVtChunkThreads + (int)threadIdx.x;
+}
+
 // Level 0: one group holding every feature, in index order.
 __global__ void k_vt_root_groups(int n, int32_t *__restrict__ groupStart, int32_t *__restrict__ tileStart)
 {
@@ -219,10 +286,18 @@ hipError_t launch_vt_level(int n, const float *keys, const float *tree, const in
 hipError_t launch_vt_regroup(int n, const int32_t *leaf, int parentBase, int nGroups, int32_t *counts,
                              int32_t *groupStart, int32_t *tileStart, int32_t *cursor, int32_t *order, hipStream_t st)
 {
+   const bool lds = nGroups <= kVtLdsBins;
    hipLaunchKernelGGL(k_vt_fill, dim3(blocks(nGroups, 256)), dim3(256), 0, st, nGroups, counts, 0);
-   if (n > 0) hipLaunchKernelGGL(k_vt_count, dim3(blocks(n, 256)), dim3(256), 0, st, n, leaf, parentBase, nGroups, counts);
+   if (n > 0 && lds)
+      hipLaunchKernelGGL(k_vt_count_lds, dim3(blocks(n, kVtChunk)), dim3(kVtChunkThreads), 0, st, n, leaf, parentBase,
+                         nGroups, counts);
+   else if (n > 0)
+      hipLaunchKernelGGL(k_vt_count, dim3(blocks(n, 256)), dim3(256), 0, st, n, leaf, parentBase, nGroups, counts);
    hipLaunchKernelGGL(k_vt_scan, dim3(1), dim3(1024), 0, st, nGroups, counts, groupStart, tileStart, cursor);
-   if (n > 0)
+   if (n > 0 && lds)
+      hipLaunchKernelGGL(k_vt_scatter_lds, dim3(blocks(n, kVtChunk)), dim3(kVtChunkThreads), 0, st, n, leaf,
+                         parentBase, nGroups, cursor, order);
+   else if (n > 0)
       hipLaunchKernelGGL(k_vt_scatter, dim3(blocks(n, 256)), dim3(256), 0, st, n, leaf, parentBase, nGroups, cursor,
                          order);
    return hipGetLastError();

@@ -21,6 +21,25 @@ def tree_nodes(depth: int, branch_factor: int) -> int:
     return sum(branch_factor ** (d + 1) for d in range(depth))
 
 
+def synthetic_tree(seed: int, depth: int, branch_factor: int, spread: float = 0.35) -> np.ndarray:
+    """Synthetic node descriptors for tests and benchmarks (no trained tree
+    ships with the reference): unit-norm, non-negative, each child a
+    perturbation of its parent, roughly what hierarchical k-means on SIFT
+    descriptors produces.  Returns (tree_nodes, 128) float32, level by level."""
+    rng = np.random.default_rng(seed)
+    tree = np.zeros((tree_nodes(depth, branch_factor), 128), np.float32)
+    parents = np.abs(rng.normal(size=(1, 128)))
+    start = 0
+    for _ in range(depth):
+        c = np.abs(parents[:, None, :] + spread * rng.normal(size=(len(parents), branch_factor, 128)))
+        c = c.reshape(-1, 128)
+        c /= np.linalg.norm(c, axis=1, keepdims=True)
+        tree[start:start + len(c)] = c
+        start += len(c)
+        parents = c
+    return tree
+
+
 class VocabularyTree:
     """A vocabulary tree resident on HIP device ``device``."""
 
@@ -154,4 +173,4 @@ def candidate_pairs(counts, words, n_leaves: int, n_best: int = 20, p_match: flo
     return [(int(a), int(b), float(s)) for (a, b), s in zip(pairs[:n.value], scores[:n.value])]
 
 
-__all__ = ["VocabularyTree", "LeafDocuments", "candidate_pairs", "tree_nodes"]
+__all__ = ["VocabularyTree", "LeafDocuments", "candidate_pairs", "tree_nodes", "synthetic_tree"]
