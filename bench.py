@@ -557,6 +557,11 @@ def run_voctree(args, v):
         a, b = t.runTimes()
         scoring.append(a)
         runs.append(b)
+    host = []
+    for _ in range(3):  # the host-buffer entry point: key upload over PCIe + run + word download
+        th = time.perf_counter()
+        t.getVisualWords(keys)
+        host.append(time.perf_counter() - th)
     t.close()
     level_avg_s = statistics.mean(scoring) / 1e3 / depth
     ops_per_launch = n * bf * 128 * 2  # one multiply and one add per term, per level
@@ -578,6 +583,7 @@ def run_voctree(args, v):
                                "(one v_mul_f32 + one v_add_f32 per term): 1/4 of the FP32 vector peak is "
                                "attainable"}
     out["run_ms"] = statistics.mean(runs)
+    out["pcie_inclusive_ms"] = statistics.median(host) * 1e3
     out["equals_oracle_sample"] = same
     out["cpu_baseline"] = {"value": cpu, "unit": "features quantised/s", "cores": threads, "kind": "port",
                            "sample": f"the first {cpu_n} keys, oracle sequential descent (OpenMP over keys)"}
