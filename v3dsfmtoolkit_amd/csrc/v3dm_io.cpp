@@ -24,6 +24,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <sstream>
 #include <string>
 #include <tuple>
@@ -273,14 +274,10 @@ struct Sqlite
 constexpr int kSqliteOk = 0;
 constexpr int kSqliteDone = 101;
 
-const Sqlite *sqlite()
+void loadSqlite(Sqlite &s)
 {
-   static Sqlite s;
-   static bool tried = false;
-   if (tried) return s.h ? &s : nullptr;
-   tried = true;
    void *h = dlopen("libsqlite3.so.0", RTLD_NOW | RTLD_LOCAL);
-   if (!h) return nullptr;
+   if (!h) return;
    bool all = true;
    auto sym = [&](auto &fn, const char *name) {
       fn = reinterpret_cast<std::remove_reference_t<decltype(fn)>>(dlsym(h, name));
@@ -300,10 +297,18 @@ const Sqlite *sqlite()
    if (!all)
    {
       dlclose(h);
-      return nullptr;
+      return;
    }
    s.h = h;
-   return &s;
+}
+
+// The library, loaded once per process (thread-safe); null if unavailable.
+const Sqlite *sqlite()
+{
+   static Sqlite s;
+   static std::once_flag once;
+   std::call_once(once, [] { loadSqlite(s); });
+   return s.h ? &s : nullptr;
 }
 
 void encodeMatch(std::vector<unsigned char> &b, int view0, int view1, const double *R, const double *T, int n,
