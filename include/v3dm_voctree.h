@@ -54,6 +54,16 @@ int v3dm_voctree_info(const v3dm_voctree *tree, int *depth, int *branchFactor, i
  * then indexes outside the tree). */
 int v3dm_voctree_quantize(v3dm_voctree *tree, int64_t n, const float *keys, int32_t *words);
 
+/* How the children are scored (the words are the same either way):
+ * V3DM_VOCTREE_AUTO (default) -- for branch factors up to 64, an FP32-MFMA pass
+ *   scores all children of 64 keys at once; only children within a proven
+ *   rounding-error bound of the best (usually none) are rescored with the
+ *   reference's sequential sum; larger branch factors use the exact kernel;
+ * V3DM_VOCTREE_EXACT_VALU -- every child with the sequential sum. */
+#define V3DM_VOCTREE_AUTO 0
+#define V3DM_VOCTREE_EXACT_VALU 1
+int v3dm_voctree_set_mode(v3dm_voctree *tree, int mode);
+
 /* The same job in three steps, for callers that keep keys resident: upload
  * n keys, run the quantisation on the device (may be repeated; no transfer),
  * fetch the words of the last run (waits). */

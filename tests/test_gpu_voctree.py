@@ -3,7 +3,9 @@ equal the CPU oracle's restatement of VocabularyTree::findBestLeafIndex
 (V3D/include/Base/v3d_voctree.h:63-99) bit for bit -- on the driver's tree
 shape (depth 3, branch factor 50, match_using_voctree.cpp:145), on other
 shapes, and on the crafted cases that separate the sequential unfused float
-sum from other summations."""
+sum from other summations -- in both scoring modes: the FP32-MFMA filter with
+exact rescoring of the children within its error bound, and exact VALU sums
+for every child."""
 import sys
 
 import numpy as np
@@ -30,6 +32,10 @@ def keys_like_sift(seed, n):
     return np.ascontiguousarray(d, np.float32)
 
 
+MODES = [VT.VocabularyTree.AUTO, VT.VocabularyTree.EXACT_VALU]
+MODE_IDS = ["mfma_filter", "exact_valu"]
+
+
 @pytest.fixture(scope="module")
 def big_tree(gpu_available):
     say("building the 3 x 50 tree")
@@ -39,8 +45,10 @@ def big_tree(gpu_available):
     t.close()
 
 
-def test_driver_tree_shape_matches_oracle(big_tree):
+@pytest.mark.parametrize("mode", MODES, ids=MODE_IDS)
+def test_driver_tree_shape_matches_oracle(big_tree, mode):
     tree, t = big_tree
+    t.setMode(mode)
     assert t.visualWordCount() == 125000
     keys = keys_like_sift(1, 20000)
     say("quantise 20000 unit keys")
@@ -54,8 +62,10 @@ def test_driver_tree_shape_matches_oracle(big_tree):
     assert 0 < scoring <= run
 
 
-def test_repeated_runs_and_smaller_jobs(big_tree):
+@pytest.mark.parametrize("mode", MODES, ids=MODE_IDS)
+def test_repeated_runs_and_smaller_jobs(big_tree, mode):
     tree, t = big_tree
+    t.setMode(mode)
     for n in (5000, 1, 65, 0, 777):
         say("job of", n, "keys")
         keys = keys_like_sift(100 + n, max(n, 1))[:n]
@@ -65,11 +75,13 @@ def test_repeated_runs_and_smaller_jobs(big_tree):
         np.testing.assert_array_equal(t.fetch(), OV.quantise(3, 50, tree, keys))
 
 
-@pytest.mark.parametrize("depth,bf", [(1, 7), (2, 300), (5, 10), (4, 13)])
-def test_other_tree_shapes(gpu_available, depth, bf):
-    say("tree shape", depth, bf)
+@pytest.mark.parametrize("mode", MODES, ids=MODE_IDS)
+@pytest.mark.parametrize("depth,bf", [(1, 7), (2, 300), (5, 10), (4, 13), (2, 33), (2, 64), (3, 32)])
+def test_other_tree_shapes(gpu_available, depth, bf, mode):
+    say("tree shape", depth, bf, mode)
     tree = kmeans_like_tree(depth * 1000 + bf, depth, bf)
     t = VT.VocabularyTree(depth, bf, tree)
+    t.setMode(mode)
     try:
         keys = keys_like_sift(depth * 7 + bf, 3000)
         np.testing.assert_array_equal(t.getVisualWords(keys), OV.quantise(depth, bf, tree, keys))
@@ -77,7 +89,8 @@ def test_other_tree_shapes(gpu_available, depth, bf):
         t.close()
 
 
-def test_crafted_rounding_and_tie_cases(gpu_available):
+@pytest.mark.parametrize("mode", MODES, ids=MODE_IDS)
+def test_crafted_rounding_and_tie_cases(gpu_available, mode):
     """Each key's answer differs between the reference's unfused sequential float
     sum and (a) the exact sum, (b) a fused multiply-add order; plus exact ties
     (first child wins) and a non-finite key (-1)."""
@@ -109,6 +122,7 @@ def test_crafted_rounding_and_tie_cases(gpu_available):
     want = OV.quantise(depth, bf, tree, keys)
     assert want[0] // bf == 1 and want[1] // bf == 3 and want[2] == 1 * bf + 1 and want[3] == -1
     t = VT.VocabularyTree(depth, bf, tree)
+    t.setMode(mode)
     try:
         np.testing.assert_array_equal(t.getVisualWords(keys), want)
         many = np.repeat(keys, 97, axis=0)  # the same cases spread over many waves and groups
@@ -131,3 +145,29 @@ def test_tree_file_round_trip(gpu_available, tmp_path):
     p.write_bytes(tree.tobytes()[:-4])
     with pytest.raises(v.V3DMError, match="needs"):
         VT.VocabularyTree(depth, bf, path=p)
+
+
+@pytest.mark.parametrize("mode", MODES, ids=MODE_IDS)
+def test_near_ties_in_the_driver_shape(gpu_available, mode):
+    """The 3 x 50 tree with, under every node, children 2k and 2k+1 made equal or
+    one ulp apart in one component, and keys drawn close to those children:
+    most keys face a near-tie that only the exact sequential sums decide
+    (the filter must send them to its exact path)."""
+    depth, bf = 3, 50
+    tree = kmeans_like_tree(77, depth, bf).copy()
+    rng = np.random.default_rng(78)
+    for start in (0, 50, 2550):
+        level = tree[start:start + (bf if start == 0 else (2500 if start == 50 else 125000))]
+        level[1::2] = level[0::2]
+        flip = rng.random(len(level) // 2) < 0.5
+        idx = np.nonzero(flip)[0] * 2 + 1
+        level[idx, 7] = np.nextafter(level[idx, 7], np.float32(2))
+    keys = np.abs(tree[2550 + rng.integers(0, 125000, 6000)] + rng.normal(0, 0.003, (6000, 128))).astype(np.float32)
+    keys /= np.linalg.norm(keys, axis=1, keepdims=True)
+    want = OV.quantise(depth, bf, tree, keys)
+    t = VT.VocabularyTree(depth, bf, tree)
+    t.setMode(mode)
+    try:
+        np.testing.assert_array_equal(t.getVisualWords(keys), want)
+    finally:
+        t.close()

@@ -131,6 +131,7 @@ _SIGNATURES = {
     "v3dm_voctree_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
     "v3dm_voctree_info": (ctypes.c_int, [ctypes.c_void_p, _c_int_p, _c_int_p, _c_int_p, _c_int_p]),
     "v3dm_voctree_quantize": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, _f32_p, _i32_p]),
+    "v3dm_voctree_set_mode": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "v3dm_voctree_upload": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, _f32_p]),
     "v3dm_voctree_run": (ctypes.c_int, [ctypes.c_void_p]),
     "v3dm_voctree_fetch": (ctypes.c_int, [ctypes.c_void_p, _i32_p]),

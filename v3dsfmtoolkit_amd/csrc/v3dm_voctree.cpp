@@ -22,10 +22,13 @@
 #include "v3dm_ctx.h"
 
 namespace v3dm {
-hipError_t launch_vt_root_groups(int n, int32_t *groupStart, int32_t *tileStart, hipStream_t st);
-hipError_t launch_vt_level(int n, const float *keys, const float *tree, const int32_t *order, const int32_t *groupStart,
-                           const int32_t *tileStart, int nGroups, int parentBase, int B, int last, int wordOffset,
-                           int32_t *leaf, hipStream_t st, hipEvent_t start, hipEvent_t stop);
+hipError_t launch_vt_root_groups(int n, int32_t *groupStart, int32_t *tileStart, uint32_t *nearTieCounts, int depth,
+                                 hipStream_t st);
+size_t vt_near_tie_bytes(int64_t n);
+hipError_t launch_vt_level(int n, const float *keys, const float *tree, const float *treeT, const float *nodeNorm,
+                           const int32_t *order, const int32_t *groupStart, const int32_t *tileStart, int nGroups,
+                           int parentBase, int B, int last, int wordOffset, int32_t *leaf, int exact, void *nearTies,
+                           uint32_t *nearTieCount, hipStream_t st, hipEvent_t start, hipEvent_t stop);
 hipError_t launch_vt_regroup(int n, const int32_t *leaf, int parentBase, int nGroups, int32_t *counts,
                              int32_t *groupStart, int32_t *tileStart, int32_t *cursor, int32_t *order, hipStream_t st);
 } // namespace v3dm
@@ -59,10 +62,15 @@ struct v3dm_voctree
    hipStream_t stream = nullptr;
    std::string err;
    float *dTree = nullptr;
+   float *dNodeNorm = nullptr; // per node an upper bound of its Euclidean norm (MFMA filter bound)
+   float *dTreeT = nullptr;    // branch factor <= 64: per sibling group, 32 blocks of (64 children x 4 elements)
+   int mode = 0;               // 0: MFMA filter where the branch factor allows, 1: exact VALU sums only
    // per-job buffers (grown)
    int64_t cap = 0, n = 0;
    float *dKeys = nullptr;
    int32_t *dLeaf = nullptr, *dOrder = nullptr;
+   void *dNearTies = nullptr;          // keys the MFMA filter leaves with several candidates (one entry per key)
+   uint32_t *dNearTieCounts = nullptr; // one counter per level
    int64_t groupCap = 0;
    int32_t *dCounts = nullptr, *dGroupStart = nullptr, *dTileStart = nullptr, *dCursor = nullptr;
    bool ran = false;
@@ -81,10 +89,11 @@ namespace {
 
 void freeJob(v3dm_voctree *t)
 {
-   for (void *p : {(void *)t->dKeys, (void *)t->dLeaf, (void *)t->dOrder})
+   for (void *p : {(void *)t->dKeys, (void *)t->dLeaf, (void *)t->dOrder, t->dNearTies})
       if (p) (void)hipFree(p);
    t->dKeys = nullptr;
    t->dLeaf = t->dOrder = nullptr;
+   t->dNearTies = nullptr;
    t->cap = 0;
 }
 
@@ -147,9 +156,44 @@ int makeTree(int device, int depth, int B, const float *nodes, v3dm_voctree **ou
    if (rc == V3DM_OK) chk(hipMalloc((void **)&t->dTree, sizeof(float) * kDim * (size_t)total), "hipMalloc(tree)");
    if (rc == V3DM_OK)
       chk(hipMemcpy(t->dTree, nodes, sizeof(float) * kDim * (size_t)total, hipMemcpyHostToDevice), "tree upload");
+   if (rc == V3DM_OK)
+   {
+      // |y| in double, rounded up to float (inf for non-finite nodes: their children are then rescored exactly)
+      std::vector<float> norms((size_t)total);
+      for (int64_t j = 0; j < total; ++j)
+      {
+         double q = 0;
+         for (int i = 0; i < kDim; ++i) q += (double)nodes[j * kDim + i] * (double)nodes[j * kDim + i];
+         const double r = std::sqrt(q);
+         float f = (float)r;
+         if (std::isfinite(r) && (double)f < r) f = std::nextafter(f, INFINITY);
+         norms[(size_t)j] = std::isfinite(r) ? f : INFINITY;
+      }
+      chk(hipMalloc((void **)&t->dNodeNorm, sizeof(float) * (size_t)total), "hipMalloc(norms)");
+      if (rc == V3DM_OK)
+         chk(hipMemcpy(t->dNodeNorm, norms.data(), sizeof(float) * (size_t)total, hipMemcpyHostToDevice), "norm upload");
+   }
+   if (rc == V3DM_OK && B <= 64)
+   {
+      // the MFMA filter's operand copy: sibling group g holds the children of node
+      // g - 1 (nodes g B .. g B + B - 1); element e of child c at ((e / 4) 64 + c) 4 + e % 4
+      const int64_t groups = starts[depth - 1] + 1;
+      std::vector<float> tt((size_t)groups * kDim * 64, 0.0f);
+      for (int64_t g = 0; g < groups; ++g)
+         for (int c = 0; c < B; ++c)
+         {
+            const float *src = nodes + (size_t)(g * B + c) * kDim;
+            float *dst = tt.data() + (size_t)g * kDim * 64;
+            for (int e = 0; e < kDim; ++e) dst[((size_t)(e / 4) * 64 + c) * 4 + e % 4] = src[e];
+         }
+      chk(hipMalloc((void **)&t->dTreeT, sizeof(float) * tt.size()), "hipMalloc(tree copy)");
+      if (rc == V3DM_OK)
+         chk(hipMemcpy(t->dTreeT, tt.data(), sizeof(float) * tt.size(), hipMemcpyHostToDevice), "tree copy upload");
+   }
    t->levelEv.assign(2 * depth, nullptr);
    for (hipEvent_t &e : t->levelEv)
       if (rc == V3DM_OK) chk(hipEventCreate(&e), "hipEventCreate");
+   if (rc == V3DM_OK) chk(hipMalloc((void **)&t->dNearTieCounts, sizeof(uint32_t) * kMaxDepth), "hipMalloc(counts)");
    if (rc == V3DM_OK) chk(hipEventCreate(&t->runStart), "hipEventCreate");
    if (rc == V3DM_OK) chk(hipEventCreate(&t->runStop), "hipEventCreate");
    if (rc != V3DM_OK)
@@ -171,6 +215,7 @@ int ensureJob(v3dm_voctree *t, int64_t n)
       VCHK(t, hipMalloc((void **)&t->dKeys, sizeof(float) * kDim * (size_t)c));
       VCHK(t, hipMalloc((void **)&t->dLeaf, sizeof(int32_t) * (size_t)c));
       VCHK(t, hipMalloc((void **)&t->dOrder, sizeof(int32_t) * (size_t)c));
+      VCHK(t, hipMalloc(&t->dNearTies, vt_near_tie_bytes(c)));
       t->cap = c;
    }
    int64_t g = 2;
@@ -230,6 +275,9 @@ void v3dm_voctree_destroy(v3dm_voctree *t)
    freeJob(t);
    freeGroups(t);
    if (t->dTree) (void)hipFree(t->dTree);
+   if (t->dNodeNorm) (void)hipFree(t->dNodeNorm);
+   if (t->dTreeT) (void)hipFree(t->dTreeT);
+   if (t->dNearTieCounts) (void)hipFree(t->dNearTieCounts);
    for (hipEvent_t e : t->levelEv)
       if (e) (void)hipEventDestroy(e);
    if (t->runStart) (void)hipEventDestroy(t->runStart);
@@ -247,6 +295,14 @@ int v3dm_voctree_info(const v3dm_voctree *t, int *depth, int *branchFactor, int 
    if (branchFactor) *branchFactor = t->B;
    if (nodes) *nodes = (int)std::min<int64_t>(t->nodes, INT32_MAX);
    if (leaves) *leaves = (int)(t->levelStart[t->depth] - t->levelStart[t->depth - 1]);
+   return V3DM_OK;
+}
+
+int v3dm_voctree_set_mode(v3dm_voctree *t, int mode)
+{
+   if (!t || (mode != V3DM_VOCTREE_AUTO && mode != V3DM_VOCTREE_EXACT_VALU))
+      return vfail(t ? &t->err : nullptr, V3DM_ERR_ARG, "voctree_set_mode: bad arguments");
+   t->mode = mode;
    return V3DM_OK;
 }
 
@@ -269,7 +325,7 @@ int v3dm_voctree_run(v3dm_voctree *t)
    VCHK(t, hipSetDevice(t->device));
    const int n = (int)t->n;
    VCHK(t, hipEventRecord(t->runStart, t->stream));
-   VCHK(t, launch_vt_root_groups(n, t->dGroupStart, t->dTileStart, t->stream));
+   VCHK(t, launch_vt_root_groups(n, t->dGroupStart, t->dTileStart, t->dNearTieCounts, t->depth, t->stream));
    for (int d = 0; d < t->depth; ++d)
    {
       const int64_t groups = groupsAt(t, d);
@@ -278,9 +334,10 @@ int v3dm_voctree_run(v3dm_voctree *t)
          VCHK(t, launch_vt_regroup(n, t->dLeaf, parentBase, (int)groups, t->dCounts, t->dGroupStart, t->dTileStart,
                                    t->dCursor, t->dOrder, t->stream));
       const bool last = d == t->depth - 1;
-      VCHK(t, launch_vt_level(n, t->dKeys, t->dTree, d == 0 ? nullptr : t->dOrder, t->dGroupStart, t->dTileStart,
-                              (int)groups, parentBase, t->B, last ? 1 : 0, (int)t->levelStart[t->depth - 1], t->dLeaf,
-                              t->stream, t->levelEv[2 * d], t->levelEv[2 * d + 1]));
+      VCHK(t, launch_vt_level(n, t->dKeys, t->dTree, t->dTreeT, t->dNodeNorm, d == 0 ? nullptr : t->dOrder, t->dGroupStart,
+                              t->dTileStart, (int)groups, parentBase, t->B, last ? 1 : 0,
+                              (int)t->levelStart[t->depth - 1], t->dLeaf, t->mode, t->dNearTies,
+                              t->dNearTieCounts + d, t->stream, t->levelEv[2 * d], t->levelEv[2 * d + 1]));
    }
    VCHK(t, hipEventRecord(t->runStop, t->stream));
    t->ran = true;

@@ -62,6 +62,11 @@ class VocabularyTree:
         if rc != V3DM_OK:
             raise V3DMError(rc, lib().v3dm_voctree_last_error(self._h).decode(errors="replace"))
 
+    AUTO, EXACT_VALU = 0, 1  # scoring modes (same words): MFMA filter + exact rescoring, or exact sums only
+
+    def setMode(self, mode: int) -> None:
+        self._check(lib().v3dm_voctree_set_mode(self._h, mode))
+
     def visualWordCount(self) -> int:
         leaves = ctypes.c_int()
         self._check(lib().v3dm_voctree_info(self._h, None, None, None, ctypes.byref(leaves)))
