@@ -17,15 +17,25 @@
 //    V3D::Vector2f included) without pulling in the V3D math headers;
 //  * the destructor releases the device state (the reference leaves it to
 //    deallocate()).
+//
+// V3D_CUDA::VocabularyTreeTraversal<BranchFactor, Depth> (reference
+// v3d_cudamatching.h:69-105, v3d_cudamatching.cpp:258-328) is here too, on the
+// library's vocabulary tree (v3dm_voctree.h).  Its leaves are those of the CPU
+// tree the voctree driver uses (V3D/include/Base/v3d_voctree.h:63-99,
+// bit-identical); the reference's own kernel sums each score in a different
+// order (v3d_cudamatching.cu:435-528), so on near-ties its leaves can differ
+// from the CPU tree's, and this class follows the CPU tree.
 #ifndef V3D_CUDA_MATCHING_H
 #define V3D_CUDA_MATCHING_H
 
 #include <cstdint>
+#include <iostream>
 #include <stdexcept>
 #include <string>
 #include <utility>
 #include <vector>
 
+#include "../v3dm_voctree.h"
 #include "../v3dmatch.h"
 
 namespace V3D_CUDA
@@ -193,6 +203,89 @@ struct SIFT_Matching
       check(rc);
       unpack(buf, count, matches);
    }
+};
+
+// Batch tree traversal on the GPU with the reference class's interface: node
+// descriptors once (setNodes), then traverseTree for up to nMaxFeatures keys
+// at a time.  leaveIds receive the leaf's node id in the whole tree (the
+// reference kernel's convention: leaf index + getNodeCount() - getLeaveCount());
+// a key with a non-finite value gets -1.  Device errors throw
+// std::runtime_error; the destructor releases the tree.
+template <int BranchFactor, int Depth>
+struct VocabularyTreeTraversal
+{
+   VocabularyTreeTraversal() : _nMaxFeatures(0), _tree(0) {}
+   ~VocabularyTreeTraversal() { v3dm_voctree_destroy(_tree); }
+
+   VocabularyTreeTraversal(VocabularyTreeTraversal const &) = delete;
+   VocabularyTreeTraversal &operator=(VocabularyTreeTraversal const &) = delete;
+
+   void allocate(int nMaxFeatures)
+   {
+      if ((nMaxFeatures % 32) != 0) // v3d_cudamatching.cpp:271-276
+      {
+         std::cerr << "VocabularyTreeTraversal::allocate() warning: nMaxFeatures should be a multiple of 32"
+                   << std::endl;
+         std::cerr << "Rounding nFeatures down." << std::endl;
+         nMaxFeatures = nMaxFeatures - (nMaxFeatures & 31);
+      }
+      _nMaxFeatures = nMaxFeatures;
+   }
+
+   void deallocate()
+   {
+      v3dm_voctree_destroy(_tree);
+      _tree = 0;
+   }
+
+   void setNodes(float const *nodes)
+   {
+      v3dm_voctree_destroy(_tree);
+      _tree = 0;
+      if (v3dm_voctree_create(0, Depth, BranchFactor, nodes, &_tree) != V3DM_OK)
+         throw std::runtime_error(std::string("VocabularyTreeTraversal: ") + v3dm_voctree_last_error(0));
+   }
+
+   void traverseTree(int nFeatures, float const *features, int *leaveIds)
+   {
+      static_assert(sizeof(int) == sizeof(std::int32_t), "int is 32 bits");
+      if (nFeatures > _nMaxFeatures) // v3d_cudamatching.cpp:311-316
+      {
+         std::cerr << "VocabularyTreeTraversal::traverseTree() warning: nFeatures is larger than nMaxFeatures."
+                   << std::endl;
+         std::cerr << "Reducing nFeatures." << std::endl;
+         nFeatures = _nMaxFeatures;
+      }
+      if (!_tree) throw std::runtime_error("VocabularyTreeTraversal: setNodes() has not been called");
+      if (nFeatures <= 0) return;
+      if (v3dm_voctree_quantize(_tree, nFeatures, features, reinterpret_cast<std::int32_t *>(leaveIds)) != V3DM_OK)
+         throw std::runtime_error(std::string("VocabularyTreeTraversal: ") + v3dm_voctree_last_error(_tree));
+      int const offset = getNodeCount() - getLeaveCount();
+      for (int i = 0; i < nFeatures; ++i)
+         if (leaveIds[i] >= 0) leaveIds[i] += offset;
+   }
+
+   static inline int getNodeCount()
+   {
+      int res = 0, len = 1;
+      for (int d = 0; d < Depth; ++d)
+      {
+         len *= BranchFactor;
+         res += len;
+      }
+      return res;
+   }
+
+   static inline int getLeaveCount()
+   {
+      int res = 1;
+      for (int d = 0; d < Depth; ++d) res *= BranchFactor;
+      return res;
+   }
+
+ protected:
+   int _nMaxFeatures;
+   v3dm_voctree *_tree;
 };
 
 } // end namespace V3D_CUDA

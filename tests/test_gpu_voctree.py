@@ -171,3 +171,24 @@ def test_near_ties_in_the_driver_shape(gpu_available, mode):
         np.testing.assert_array_equal(t.getVisualWords(keys), want)
     finally:
         t.close()
+
+
+def test_cpp_dropin_traversal_matches_oracle(big_tree, tmp_path):
+    """V3D_CUDA::VocabularyTreeTraversal<50, 3> from include/CUDA/v3d_cudamatching.h
+    (tests/cpp/voctree_check.cpp): leaf node ids = the CPU tree's words + the
+    first leaf's node id (v3d_cudamatching.cu:514-521), in batches, with the
+    reference's rounding warning."""
+    import os
+    import subprocess
+    tree, _ = big_tree
+    keys = keys_like_sift(55, 10000)
+    tree.tofile(tmp_path / "tree.bin")
+    keys.tofile(tmp_path / "keys.bin")
+    exe = os.path.join(os.path.dirname(__file__), "cpp", "voctree_check")
+    say("voctree_check")
+    r = subprocess.run([exe, str(tmp_path / "tree.bin"), str(tmp_path / "keys.bin"), str(len(keys)),
+                        str(tmp_path / "leaves.bin")], capture_output=True, text=True, timeout=100)
+    assert r.returncode == 0, r.stderr
+    assert "nMaxFeatures should be a multiple of 32" in r.stderr
+    leaves = np.fromfile(tmp_path / "leaves.bin", np.int32)
+    np.testing.assert_array_equal(leaves, OV.quantise(3, 50, tree, keys) + 2550)
