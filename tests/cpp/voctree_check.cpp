@@ -2,8 +2,8 @@
 // as a caller of the reference class would: allocate, setNodes, traverseTree
 // in batches.  Usage: voctree_check <tree file> <keys file> <n keys> <out file>
 // (raw float32 node descriptors and keys, int32 leaf ids out), tree shape
-// branch factor 50, depth 3, batches of 4000 keys (not a multiple of 32, to
-// exercise the rounding warning path).
+// branch factor 50, depth 3, allocate(3999) (not a multiple of 32: the
+// reference's rounding warning, 3968 keys per batch).
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -27,7 +27,7 @@ int main(int argc, char **argv)
    try
    {
       Traversal t;
-      t.allocate(4000); // rounded down to 3968 with the reference's warning
+      t.allocate(3999); // rounded down to 3968 with the reference's warning
       t.setNodes(nodes.data());
       for (int i = 0; i < n; i += 3968)
       {
