@@ -84,10 +84,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run a plain device copy of this size (scales memory counters)")
-    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5", "voctree"], default="c2",
+    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5", "voctree", "sequence"], default="c2",
                     help="c2: single pair 8192^2 (default line); c3: 256 uneven pairs in one batch (+ float "
                          "variant); c4: learned path on 64 pairs of 16384^2; c5: single pair 65536^2; voctree: "
-                         "vocabulary-tree quantisation of 64 views x 7168 features (depth 3, branch factor 50)")
+                         "vocabulary-tree quantisation of 64 views x 7168 features (depth 3, branch factor 50); "
+                         "sequence: the match_sequence counterpart on 64 feature files, radius 3")
     ap.add_argument("--pairs", type=int, default=0, help="number of pairs for c3 / c4 (0: the workload's own)")
     ap.add_argument("--scan-variant", type=int, default=-1,
                     help="single-pair int8 scan variant (-1: the library measures the fastest)")
@@ -103,7 +104,8 @@ def main():
     import v3dsfmtoolkit_amd as v
 
     if args.workload != "c2":
-        out = {"c3": run_c3, "c4": run_c4, "c5": run_c5, "voctree": run_voctree}[args.workload](args, v)
+        out = {"c3": run_c3, "c4": run_c4, "c5": run_c5, "voctree": run_voctree,
+               "sequence": run_sequence}[args.workload](args, v)
         print(json.dumps(out), flush=True)
         return
     run_c2(args, v)
@@ -595,6 +597,94 @@ def run_voctree(args, v):
     out["equals_oracle_sample"] = same
     out["cpu_baseline"] = {"value": cpu, "unit": "features quantised/s", "cores": threads, "kind": "port",
                            "sample": f"the first {cpu_n} keys, oracle sequential descent (OpenMP over keys)"}
+    return out
+
+
+def run_sequence(args, v):
+    """The putative-matching stage of a whole sequence run through the native
+    counterpart driver (v3dm_match_sequence): 64 views of 7168 SIFT-like
+    features in V3DBIN files, each view sharing 40 % of its features (noisy
+    copies, sigma 0.02) with the previous one, radius 3 (each view against the
+    next five), MATCHING_MAX_FEATURES 7168.  A step is one application run:
+    read every file, upload every view once, match all pairs as one batched
+    job, write the SQLite rows."""
+    import re
+    import subprocess
+    import tempfile
+    from oracle import oracle as O
+    from v3dsfmtoolkit_amd import drivers as D
+    from v3dsfmtoolkit_amd import features as F
+    views, per, radius = args.pairs or 64, 7168, 3
+    rng = np.random.default_rng(77)
+    tmp = tempfile.mkdtemp(prefix="v3dm_seq_")
+    progress(f"writing {views} feature files to {tmp}")
+    prev = None
+    descs = []
+    for i in range(views):
+        x = v.synth_pair(v.synth_seed(8, i), per, 0, u8=False, f32=True)["Lf"].astype(np.float64)
+        if prev is not None:
+            k = int(0.4 * per)
+            dst, src = rng.choice(per, k, replace=False), rng.choice(per, k, replace=False)
+            w = np.maximum(prev[src] + rng.normal(0, 0.02, (k, 128)), 0)
+            w /= np.linalg.norm(w, axis=1, keepdims=True)
+            w = np.minimum(w, 0.2)
+            x[dst] = w / np.linalg.norm(w, axis=1, keepdims=True)
+        prev = x
+        q = (np.clip(np.rint(x * 512), 0, 255) / 512).astype(np.float32)
+        descs.append(q)
+        F.write_features(os.path.join(tmp, f"v{i:04d}.features"), q, ids=np.arange(per, dtype=np.int32),
+                         positions=rng.uniform(0, 2000, (per, 2)).astype(np.float32))
+    with open(os.path.join(tmp, "list.txt"), "w") as f:
+        f.write("".join(f"v{i:04d}\n" for i in range(views)))
+    with open(os.path.join(tmp, "run.conf"), "w") as f:
+        f.write("MATCHING_MAX_FEATURES 7168\nMATCHING_SCORE_RATIO 0.8\nMINIMUM_RAW_CORRESPONDENCES 100\n")
+    app = os.path.join(ROOT, "v3dsfmtoolkit_amd", "bin", "v3dm_match_sequence")
+    pairs = D.sequence_pairs(views, radius)
+    steps = max(1, min(args.steps, 5))
+    runs = []
+    for it in range(steps + 1):
+        t0 = time.perf_counter()
+        r = subprocess.run([app, "--quiet", "--db", os.path.join(tmp, "out.db"), "list.txt", "run.conf", str(radius)],
+                           cwd=tmp, capture_output=True, text=True, timeout=300)
+        wall = time.perf_counter() - t0
+        if r.returncode != 0:
+            raise RuntimeError(r.stderr)
+        m = re.search(r"matched (\d+) view pairs \((\d+) descriptor pairs, (\w+) pool\) in ([\d.]+) ms after a "
+                      r"([\d.]+) ms upload; (\d+) putative matches; (\d+) pairs", r.stdout)
+        if it > 0:  # the first run warms the page cache and the GPU
+            runs.append((wall, float(m.group(4)), float(m.group(5)), int(m.group(6)), int(m.group(7)),
+                         int(m.group(2)), m.group(3)))
+        progress(f"run {it}: {wall:.3f} s wall, matching {m.group(4)} ms")
+    wall = statistics.median(x[0] for x in runs)
+    match_ms = statistics.median(x[1] for x in runs)
+    upload_ms = statistics.median(x[2] for x in runs)
+    n_pairs, scored, pool = len(pairs), runs[0][5], runs[0][6]
+    # CPU baseline and parity: the oracle's blocked rule on four of the pairs
+    threads = len(os.sched_getaffinity(0))
+    sample = [0, 1, len(pairs) // 2, len(pairs) - 1]
+    t = time.perf_counter()
+    sample_matches = 0
+    for p in sample:
+        a, b = pairs[p]
+        _, _, tab, _ = O.find_u8_blocked(D.descriptors_to_u8(descs[a]), D.descriptors_to_u8(descs[b]), 0.9, 0.8,
+                                         threads=threads)
+        sample_matches += int((tab >= 0).sum())
+    cpu_s = (time.perf_counter() - t) / len(sample)
+    out = _common("image pairs matched/s", n_pairs / (match_ms / 1e3), "image pairs/s", len(runs), 1,
+                  match_ms / 1e3, "int8",
+                  {"workload": f"match_sequence counterpart: {views} views x {per} SIFT-like features in V3DBIN files, "
+                               f"40% shared with the previous view (sigma 0.02), radius {radius}, minScore 0.9, "
+                               "minRatio 0.8", "views": views, "view_pairs": n_pairs,
+                   "descriptor_pairs": scored, "pool": pool})
+    out["descriptor_pairs_per_s"] = scored / (match_ms / 1e3)
+    out["app_wall_s"] = wall
+    out["app_image_pairs_per_s"] = n_pairs / wall
+    out["upload_ms"] = upload_ms
+    out["putative_matches"] = runs[0][3]
+    out["stored_pairs"] = runs[0][4]
+    out["cpu_baseline"] = {"value": 1.0 / cpu_s, "unit": "image pairs/s", "cores": threads, "kind": "port",
+                           "sample": f"{len(sample)} of the view pairs, oracle blocked rule (exact int32 dots)",
+                           "sample_matches": sample_matches}
     return out
 
 

@@ -118,6 +118,10 @@ struct MatchOptions
    int nMinRawCorrs = 100;
    int device = 0;
    int pairsPerJob = 4096;
+   // scan tile variant of the batched jobs: 1 = 256 x 1024 tiles, the fastest on
+   // large batches; -1 = time every variant on the first job (worth it only with
+   // the V3DM_TUNE_CACHE file, since each run is a new process)
+   int variant = 1;
    bool quiet = false;
    std::string dbPath; // empty: no database
 };
@@ -146,6 +150,7 @@ inline int matchPairs(std::vector<View> &views, const std::vector<int32_t> &pair
 
    v3dm_ctx *ctx = nullptr;
    check(v3dm_create(o.device, &ctx), "v3dm_create");
+   check(v3dm_batch_set_variant(ctx, o.variant), "v3dm_batch_set_variant", ctx);
    const auto t0 = std::chrono::steady_clock::now();
    if (useU8)
       check(v3dm_batch_set_images_u8(ctx, nViews, counts.data(), allU8.data()), "upload", ctx);

@@ -27,7 +27,8 @@
 // (:340-346).  The per-pair log lines follow the reference's.
 //
 // Options: --db PATH (default putative_matches.db), --no-db, --device N,
-//          --pairs-per-job N (default 4096), --quiet.
+//          --pairs-per-job N (default 4096), --autotune (time the scan tile
+//          variants on the first job instead of 256 x 1024 tiles), --quiet.
 #include <cstdlib>
 #include <fstream>
 #include <iostream>
@@ -45,7 +46,7 @@ void usage(const char *argv0)
 {
    std::cerr << "Usage: " << argv0 << " [options] <image list file> <file.conf> <radius>\n"
              << "       " << argv0 << " [options] --pairs <pairs file> <image list file> <file.conf>\n"
-             << "options: --db PATH | --no-db | --device N | --pairs-per-job N | --quiet" << std::endl;
+             << "options: --db PATH | --no-db | --device N | --pairs-per-job N | --autotune | --quiet" << std::endl;
 }
 
 // "a b" or "a -- b;" per line; other lines ("graph {", "}") are skipped.
@@ -95,6 +96,7 @@ int main(int argc, char **argv)
       else if (a == "--pairs-per-job") o.pairsPerJob = std::atoi(value());
       else if (a == "--pairs") pairsFile = value();
       else if (a == "--quiet") o.quiet = true;
+      else if (a == "--autotune") o.variant = -1;
       else if (a.size() > 1 && a[0] == '-' && a[1] == '-')
       {
          usage(argv[0]);

@@ -20,7 +20,8 @@
 //
 // Options: --db PATH (default putative_matches.db), --no-db, --dot PATH,
 //          --tree-shape DEPTH BRANCH (default 3 50), --device N,
-//          --pairs-per-job N (default 4096), --keys-per-job N (default 4194304), --quiet.
+//          --pairs-per-job N (default 4096), --keys-per-job N (default 4194304),
+//          --autotune (time the scan tile variants on the first job), --quiet.
 #include <algorithm>
 #include <cstdlib>
 #include <fstream>
@@ -40,7 +41,7 @@ void usage(const char *argv0)
 {
    std::cerr << "Usage: " << argv0 << " [options] <vocabulary tree file> <image list file> <file.conf>\n"
              << "options: --db PATH | --no-db | --dot PATH | --tree-shape DEPTH BRANCH | --device N |\n"
-             << "         --pairs-per-job N | --keys-per-job N | --quiet" << std::endl;
+             << "         --pairs-per-job N | --keys-per-job N | --autotune | --quiet" << std::endl;
 }
 
 } // namespace
@@ -76,6 +77,7 @@ int main(int argc, char **argv)
       else if (a == "--pairs-per-job") o.pairsPerJob = std::atoi(value());
       else if (a == "--keys-per-job") keysPerJob = std::atoll(value());
       else if (a == "--quiet") o.quiet = true;
+      else if (a == "--autotune") o.variant = -1;
       else if (a.size() > 1 && a[0] == '-' && a[1] == '-')
       {
          usage(argv[0]);
