@@ -84,11 +84,13 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run a plain device copy of this size (scales memory counters)")
-    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5", "voctree", "sequence"], default="c2",
+    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5", "voctree", "sequence", "voctree-driver"],
+                    default="c2",
                     help="c2: single pair 8192^2 (default line); c3: 256 uneven pairs in one batch (+ float "
                          "variant); c4: learned path on 64 pairs of 16384^2; c5: single pair 65536^2; voctree: "
                          "vocabulary-tree quantisation of 64 views x 7168 features (depth 3, branch factor 50); "
-                         "sequence: the match_sequence counterpart on 64 feature files, radius 3")
+                         "sequence: the match_sequence counterpart on 64 feature files, radius 3; voctree-driver: "
+                         "the match_using_voctree counterpart on the same files")
     ap.add_argument("--pairs", type=int, default=0, help="number of pairs for c3 / c4 (0: the workload's own)")
     ap.add_argument("--scan-variant", type=int, default=-1,
                     help="single-pair int8 scan variant (-1: the library measures the fastest)")
@@ -105,7 +107,7 @@ def main():
 
     if args.workload != "c2":
         out = {"c3": run_c3, "c4": run_c4, "c5": run_c5, "voctree": run_voctree,
-               "sequence": run_sequence}[args.workload](args, v)
+               "sequence": run_sequence, "voctree-driver": run_voctree_driver}[args.workload](args, v)
         print(json.dumps(out), flush=True)
         return
     run_c2(args, v)
@@ -608,13 +610,20 @@ def run_sequence(args, v):
     next five), MATCHING_MAX_FEATURES 7168.  A step is one application run:
     read every file, upload every view once, match all pairs as one batched
     job, write the SQLite rows."""
-    import re
-    import subprocess
-    import tempfile
-    from oracle import oracle as O
     from v3dsfmtoolkit_amd import drivers as D
-    from v3dsfmtoolkit_amd import features as F
     views, per, radius = args.pairs or 64, 7168, 3
+    tmp, descs = _sequence_files(v, views, per)
+    app = os.path.join(ROOT, "v3dsfmtoolkit_amd", "bin", "v3dm_match_sequence")
+    return _run_driver(v, args, tmp, descs, [app, "--quiet", "--db", os.path.join(tmp, "out.db"), "list.txt",
+                                             "run.conf", str(radius)], D.sequence_pairs(views, radius).tolist(),
+                       f"match_sequence counterpart: {views} views x {per} SIFT-like features in V3DBIN files, 40% "
+                       f"shared with the previous view (sigma 0.02), radius {radius}, minScore 0.9, minRatio 0.8")
+
+
+def _sequence_files(v, views, per):
+    """Synthetic run: views of SIFT-like u/512 features, 40 % shared with the previous view."""
+    import tempfile
+    from v3dsfmtoolkit_amd import features as F
     rng = np.random.default_rng(77)
     tmp = tempfile.mkdtemp(prefix="v3dm_seq_")
     progress(f"writing {views} feature files to {tmp}")
@@ -637,28 +646,40 @@ def run_sequence(args, v):
     with open(os.path.join(tmp, "list.txt"), "w") as f:
         f.write("".join(f"v{i:04d}\n" for i in range(views)))
     with open(os.path.join(tmp, "run.conf"), "w") as f:
-        f.write("MATCHING_MAX_FEATURES 7168\nMATCHING_SCORE_RATIO 0.8\nMINIMUM_RAW_CORRESPONDENCES 100\n")
-    app = os.path.join(ROOT, "v3dsfmtoolkit_amd", "bin", "v3dm_match_sequence")
-    pairs = D.sequence_pairs(views, radius)
+        f.write("MATCHING_MAX_FEATURES 7168\nMATCHING_SCORE_RATIO 0.8\nMINIMUM_RAW_CORRESPONDENCES 100\n"
+                "POTENTIAL_VIEWS_COUNT 5\nVISUAL_WORD_MATCH_PROPABILITY 0.1\n")
+    return tmp, descs
+
+
+def _run_driver(v, args, tmp, descs, cmd, pairs, workload):
+    """Runs a counterpart driver application steps + 1 times (the first warms caches)
+    and reports its matching stage, its whole run and the oracle on four pairs."""
+    import re
+    import subprocess
+    from oracle import oracle as O
+    from v3dsfmtoolkit_amd import drivers as D
+    views, per = len(descs), len(descs[0])
     steps = max(1, min(args.steps, 5))
     runs = []
     for it in range(steps + 1):
         t0 = time.perf_counter()
-        r = subprocess.run([app, "--quiet", "--db", os.path.join(tmp, "out.db"), "list.txt", "run.conf", str(radius)],
-                           cwd=tmp, capture_output=True, text=True, timeout=300)
+        r = subprocess.run(cmd, cwd=tmp, capture_output=True, text=True, timeout=300)
         wall = time.perf_counter() - t0
         if r.returncode != 0:
             raise RuntimeError(r.stderr)
         m = re.search(r"matched (\d+) view pairs \((\d+) descriptor pairs, (\w+) pool\) in ([\d.]+) ms after a "
                       r"([\d.]+) ms upload; (\d+) putative matches; (\d+) pairs", r.stdout)
+        q = re.search(r"quantised (\d+) features of (\d+) views in ([\d.]+) ms; (\d+) candidate pairs", r.stdout)
         if it > 0:  # the first run warms the page cache and the GPU
             runs.append((wall, float(m.group(4)), float(m.group(5)), int(m.group(6)), int(m.group(7)),
-                         int(m.group(2)), m.group(3)))
+                         int(m.group(2)), m.group(3), float(q.group(3)) if q else None, int(q.group(4)) if q else None))
         progress(f"run {it}: {wall:.3f} s wall, matching {m.group(4)} ms")
     wall = statistics.median(x[0] for x in runs)
     match_ms = statistics.median(x[1] for x in runs)
     upload_ms = statistics.median(x[2] for x in runs)
     n_pairs, scored, pool = len(pairs), runs[0][5], runs[0][6]
+    if runs[0][8] is not None:
+        assert runs[0][8] == n_pairs, (runs[0][8], n_pairs)
     # CPU baseline and parity: the oracle's blocked rule on four of the pairs
     threads = len(os.sched_getaffinity(0))
     sample = [0, 1, len(pairs) // 2, len(pairs) - 1]
@@ -672,10 +693,10 @@ def run_sequence(args, v):
     cpu_s = (time.perf_counter() - t) / len(sample)
     out = _common("image pairs matched/s", n_pairs / (match_ms / 1e3), "image pairs/s", len(runs), 1,
                   match_ms / 1e3, "int8",
-                  {"workload": f"match_sequence counterpart: {views} views x {per} SIFT-like features in V3DBIN files, "
-                               f"40% shared with the previous view (sigma 0.02), radius {radius}, minScore 0.9, "
-                               "minRatio 0.8", "views": views, "view_pairs": n_pairs,
-                   "descriptor_pairs": scored, "pool": pool})
+                  {"workload": workload, "views": views, "view_pairs": n_pairs, "descriptor_pairs": scored,
+                   "pool": pool})
+    if runs[0][7] is not None:
+        out["quantise_ms"] = statistics.median(x[7] for x in runs)
     out["descriptor_pairs_per_s"] = scored / (match_ms / 1e3)
     out["app_wall_s"] = wall
     out["app_image_pairs_per_s"] = n_pairs / wall
@@ -686,6 +707,26 @@ def run_sequence(args, v):
                            "sample": f"{len(sample)} of the view pairs, oracle blocked rule (exact int32 dots)",
                            "sample_matches": sample_matches}
     return out
+
+
+def run_voctree_driver(args, v):
+    """The match_using_voctree counterpart (v3dm_match_using_voctree) on the same
+    64 synthetic feature files: GPU quantisation of every feature on a
+    depth-3, branch-50 tree (synthetic nodes), binomial candidate choice
+    (POTENTIAL_VIEWS_COUNT 5), then the batched matching stage."""
+    from oracle import voctree as OV
+    from v3dsfmtoolkit_amd import voctree as VT
+    views, per = args.pairs or 64, 7168
+    tmp, descs = _sequence_files(v, views, per)
+    tree = VT.synthetic_tree(1, 3, 50)
+    tree.tofile(os.path.join(tmp, "tree.bin"))
+    words = OV.quantise(3, 50, tree, np.concatenate(descs))
+    pairs = [(a, b) for a, b, _ in OV.candidate_pairs([per] * views, words, 125000, 5, float(np.float32(0.1)))]
+    app = os.path.join(ROOT, "v3dsfmtoolkit_amd", "bin", "v3dm_match_using_voctree")
+    return _run_driver(v, args, tmp, descs, [app, "--quiet", "--db", os.path.join(tmp, "out.db"), "tree.bin",
+                                             "list.txt", "run.conf"], pairs,
+                       f"match_using_voctree counterpart: {views} views x {per} SIFT-like features in V3DBIN files, "
+                       "vocabulary tree depth 3 x branch 50, POTENTIAL_VIEWS_COUNT 5, minScore 0.9, minRatio 0.8")
 
 
 if __name__ == "__main__":
