@@ -236,7 +236,21 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(2)))
    store(R0, 0);
    fetch(R0, 2);
 
-   v4i Bc[4]; // fragments of the next step to compute
+   // Software pipeline over the steps of the tile: the MFMAs of the next step
+   // (also across a group boundary) are issued before the survivor test of the
+   // current one, so the test's VALU work overlaps the matrix pipeline instead
+   // of waiting for its result.
+   auto mfma_step = [&](const v4i(&Bf)[4], v16i &x0, v16i &x1) {
+      x0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[0][0], Bf[0], C[0], 0, 0, 0);
+      x1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[1][0], Bf[0], C[1], 0, 0, 0);
+#pragma unroll
+      for (int k = 1; k < 4; ++k)
+      {
+         x0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[0][k], Bf[k], x0, 0, 0, 0);
+         x1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[1][k], Bf[k], x1, 0, 0, 0);
+      }
+   };
+   v16i p0, p1; // scores of the step in flight
    auto group = [&](auto slotc, int g) {
       // ring slot `slotc` holds group g+1: stage it, then refill with group g+3
       store(slotc, (g + 1) % kThirds);
@@ -247,35 +261,32 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(2)))
       const int32_t *sc = sC[g % kThirds];
       if (g == 0)
       {
+         v4i B0[4];
 #pragma unroll
-         for (int k = 0; k < 4; ++k) Bc[k] = *reinterpret_cast<const v4i *>(sb + fragoff[k]);
+         for (int k = 0; k < 4; ++k) B0[k] = *reinterpret_cast<const v4i *>(sb + fragoff[k]);
+         mfma_step(B0, p0, p1);
       }
 #pragma unroll
       for (int s = 0; s < kGroup; ++s)
       {
-         // Prefetch the next step's fragments; after the barrier of group g the
-         // stage of group g+1 is complete too (written before that barrier).
+         // Fragments of the next step; after the barrier of group g the stage
+         // of group g+1 is complete too (written before that barrier).
          v4i Bn[4];
+         const bool more = s + 1 < kGroup || g + 1 < ngroups;
          if (s + 1 < kGroup)
          {
 #pragma unroll
             for (int k = 0; k < 4; ++k) Bn[k] = *reinterpret_cast<const v4i *>(sb + (s + 1) * kStageBytes + fragoff[k]);
          }
-         else if (g + 1 < ngroups)
+         else if (more)
          {
             const int8_t *sn = sB[(g + 1) % kThirds];
 #pragma unroll
             for (int k = 0; k < 4; ++k) Bn[k] = *reinterpret_cast<const v4i *>(sn + fragoff[k]);
          }
+         const v16i acc0 = p0, acc1 = p1;
+         if (more) mfma_step(Bn, p0, p1);
          const int cs = sc[s * kStepCols + lrow];
-         v16i acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[0][0], Bc[0], C[0], 0, 0, 0);
-         v16i acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[1][0], Bc[0], C[1], 0, 0, 0);
-#pragma unroll
-         for (int k = 1; k < 4; ++k)
-         {
-            acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[0][k], Bc[k], acc0, 0, 0, 0);
-            acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[1][k], Bc[k], acc1, 0, 0, 0);
-         }
          const int m = max(max16(acc0), max16(acc1));
          const int lim = thr - cs; // score > thr  <=>  acc > thr - 128*sum(u_right)
          if (__builtin_expect(__any(m > lim), 0))
@@ -304,11 +315,6 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(2)))
                   nsurv += __builtin_popcountll(ball); // counts past survCap: region overflowed
                }
             }
-         }
-         if (s + 1 < kGroup || g + 1 < ngroups)
-         {
-#pragma unroll
-            for (int k = 0; k < 4; ++k) Bc[k] = Bn[k];
          }
       }
    };
