@@ -106,7 +106,9 @@ __device__ inline int select32(const v16i &a0, const v16i &a1, int v)
 // fetched into registers.  One barrier per group; the third written at group
 // g was last read at group g-2, which every wave finished before the
 // barrier of group g-1.  Within a group the waves run free, and each wave
-// reads the fragments of step s+1 while its MFMAs of step s execute.
+// reads the fragments of step s+1 and issues its MFMAs before it tests the
+// scores of step s (two accumulator sets), so the test overlaps the matrix
+// pipeline.
 // LDS unit layout: 16-byte unit p = row * 8 + (chunk ^ swz(row)),
 // swz(r) = (r ^ r >> 3) & 7, so the 32 rows of one fragment read (same
 // chunk) fall in distinct bank groups for 8- and 16-lane LDS service.
