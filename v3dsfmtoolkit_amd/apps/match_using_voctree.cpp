@@ -132,6 +132,10 @@ int main(int argc, char **argv)
          int i1 = i0;
          int64_t n = 0;
          keys.clear();
+         int64_t jobKeys = 0;
+         for (int i = i0; i < nViews && (i == i0 || jobKeys + views[i].count <= keysPerJob); ++i)
+            jobKeys += views[i].count;
+         keys.reserve((size_t)jobKeys * kDim); // one allocation for the job's keys
          while (i1 < nViews && (i1 == i0 || n + views[i1].count <= keysPerJob))
          {
             keys.insert(keys.end(), views[i1].desc.begin(), views[i1].desc.end());
