@@ -131,6 +131,36 @@ def test_crafted_rounding_and_tie_cases(gpu_available, mode):
         t.close()
 
 
+@pytest.mark.parametrize("mode", MODES, ids=MODE_IDS)
+def test_extreme_magnitudes(gpu_available, mode):
+    """Keys and nodes far from unit scale, where the filter's bound leans on
+    its absolute terms: keys whose squares underflow (the norm bound's 2^-59
+    term), subnormal components and a subnormal child (inputs the matrix core
+    may flush), the largest finite node values, and products that overflow
+    (non-finite scores: every child is rescored)."""
+    depth, bf = 2, 16
+    rng = np.random.default_rng(91)
+    tree = rng.normal(0, 0.1, (VT.tree_nodes(depth, bf), 128)).astype(np.float32)
+    tree[5] *= np.float32(1e-39 / 0.1)        # a subnormal child
+    tree[bf + 7] = np.float32(3.4e38)          # near the largest finite float
+    base = np.abs(rng.normal(0, 0.1, (64, 128))).astype(np.float32)
+    keys = []
+    for scale in (1e-30, 1e-22, 1e-19, 1e-3, 1.0, 1e18, 1e19, 3.0e38):
+        keys.append((base * np.float32(scale)).astype(np.float32))
+    sub = base.copy()
+    sub[:, ::3] = np.float32(1e-40)            # subnormal components
+    keys.append(sub)
+    keys = np.concatenate(keys).astype(np.float32)
+    want = OV.quantise(depth, bf, tree, keys)
+    t = VT.VocabularyTree(depth, bf, tree)
+    t.setMode(mode)
+    try:
+        say("extreme magnitudes", mode, len(keys))
+        np.testing.assert_array_equal(t.getVisualWords(keys), want)
+    finally:
+        t.close()
+
+
 def test_tree_file_round_trip(gpu_available, tmp_path):
     depth, bf = 2, 9
     tree = kmeans_like_tree(5, depth, bf)

@@ -117,7 +117,8 @@ __global__ __launch_bounds__(64) void k_vt_level(const float *__restrict__ keys,
 // child c and t_c the MFMA sum.  Any summation of the 128 products with every
 // product and addition rounded to FP32 (nearest or toward zero, relative error
 // <= 2^-23) is within 129 * 2^-23 * sum|x_i y_i| of the exact dot product, and
-// s_c within 129 * 2^-24 of it; with sum|x_i y_i| <= |x| |y_c|,
+// s_c within 129 * 2^-24 of it; with sum|x_i y_i| <= |x| |y_c| (|x|, |y_c|: upper
+// bounds of the norms, vt_norm_bound and the stored node norms),
 //     |t_c - s_c| <= E_c = 2^-14 |x| |y_c| + 2^-122 (|x| + |y_c|) + 2^-100
 // (2.6x margin on the relative term; the middle term covers subnormal inputs
 // the matrix core may flush, sum over flushed i of |x_i y_i| <= 2^-126 sqrt(128)
@@ -129,6 +130,12 @@ __global__ __launch_bounds__(64) void k_vt_level(const float *__restrict__ keys,
 // Non-finite scores or norms, or LB <= -1e30f, make every child a candidate.
 
 typedef float v16f __attribute__((ext_vector_type(16)));
+
+// An upper bound of |x| from the FP32 sum of squares ss of its elements: the
+// sum is within 2^-17 (relative) of |x|^2 plus 128 * 2^-126 for squares that
+// underflow, so sqrt(ss) (1 + 2^-15) + 2^-59 >= |x| (keys too small for their
+// squares to register still get a bound that covers their scores' errors).
+__device__ inline float vt_norm_bound(float ss) { return sqrtf(ss) * (1.0f + 0x1p-15f) + 0x1p-59f; }
 
 struct VtNearTie
 {
@@ -234,7 +241,7 @@ __global__ __launch_bounds__(64) void k_vt_level_mfma(const float *__restrict__ 
       // candidate mask of key 32 kb + lj (both lane halves end with the same mask):
       // first the lower bounds t_c - E_c, turning each accumulator into its upper
       // bound t_c + E_c on the way (-inf for padding children), then the mask
-      const float nx = sqrtf(ss + __shfl_xor(ss, 32, 64));
+      const float nx = vt_norm_bound(ss + __shfl_xor(ss, 32, 64));
       float lbs = -INFINITY;
       bool bad = !isfinite(nx);
 #pragma unroll
