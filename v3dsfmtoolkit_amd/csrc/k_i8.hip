@@ -49,10 +49,6 @@ typedef int v16i __attribute__((ext_vector_type(16)));
 
 constexpr int kBias = 1 << 21;      // 128^3
 
-
-
-
-
 __device__ __forceinline__ void survivor(const ScanArgs &a, const PairDesc &pd, uint32_t epoch, int il, int jr,
                                          int32_t score)
 {
@@ -69,7 +65,6 @@ __device__ __forceinline__ void survivor(const ScanArgs &a, const PairDesc &pd, 
    top2_insert(a.top1R + pd.slotR + jr, a.top2R + pd.slotR + jr, pack_key(epoch, (uint32_t)score, (uint32_t)il));
 }
 
-
 __device__ inline int max16(const v16i &x)
 {
    int m0 = max(max(x[0], x[1]), x[2]);
@@ -79,15 +74,6 @@ __device__ inline int max16(const v16i &x)
    int m4 = max(max(x[12], x[13]), x[14]);
    return max(max(max(m0, m1), max(m2, m3)), max(m4, x[15]));
 }
-
-// acc0/acc1 register v (0..31, block v / 16).  A variable-index vector
-// extract stays in registers (indexed register moves), no scratch.
-__device__ inline int select32(const v16i &a0, const v16i &a1, int v)
-{
-   const int x0 = a0[v & 15], x1 = a1[v & 15];
-   return v < 16 ? x0 : x1;
-}
-
 
 // Operand packing.  Descriptor byte e of a row goes to lane half h = e / 64,
 // MFMA step s = (e % 64) / 16, byte e % 16 of the lane's 16-byte operand.
@@ -116,8 +102,6 @@ __device__ inline int select32(const v16i &a0, const v16i &a1, int v)
 constexpr int kStageBytes = kStepCols * kDim;      // 4 KiB
 constexpr int kGroup = 4;                          // steps per barrier
 constexpr int kGroupBytes = kGroup * kStageBytes;  // 16 KiB
-
-
 
 // One workgroup per tile, or (a.persistent) resident workgroups that take
 // tiles blockIdx.x, then gridDim.x + atomicAdd(tileCounter) until the tiles
@@ -377,7 +361,6 @@ __global__ __launch_bounds__(256) void k_i8_insert(ScanArgs a)
    }
 }
 
-
 __device__ inline float key_value(uint64_t k, uint32_t epoch)
 {
    return key_epoch(k) == epoch ? (float)key_score(k) * (1.0f / 262144.0f) : -1e30f;
@@ -523,7 +506,6 @@ hipError_t launch_i8_insert(const ScanArgs &a, hipStream_t st)
    return hipGetLastError();
 }
 
-
 hipError_t launch_i8_finalize(const FinalizeArgs &a, hipStream_t st)
 {
    if (a.nBlocks <= 0) return hipSuccess;
@@ -550,7 +532,5 @@ hipError_t launch_mfma_probe(int32_t *out, hipStream_t st)
    hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, st, out);
    return hipGetLastError();
 }
-
-
 
 } // namespace v3dm
