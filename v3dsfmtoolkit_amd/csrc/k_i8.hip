@@ -279,26 +279,34 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(2)))
          {
             // rare: append every surviving score of the wave to its region
             // (plain stores, nothing waits on them); k_i8_insert applies them.
+            // Padding rows and columns never pass (kPadRowSum), so there are
+            // no bounds to check here; blocks of 4 registers are skipped
+            // whole when none of their scores passes.
             const int jrel = (g * kGroup + s) * kStepCols + lrow;
-            const bool colOk = c0 + jrel < pd.nR;
 #pragma unroll
             for (int b = 0; b < 2; ++b)
             {
                const v16i &acc = b == 0 ? acc0 : acc1;
-               if (!__any(max16(acc) > lim)) continue;
 #pragma unroll
-               for (int v = 0; v < 16; ++v)
+               for (int q = 0; q < 4; ++q)
                {
-                  const int irel = 32 * b + 8 * (v >> 2) + 4 * half + (v & 3);
-                  const bool h = acc[v] > lim && colOk && l0 + irel < pd.nL;
-                  const uint64_t ball = __ballot(h);
-                  if (ball == 0) continue;
-                  const int slot = nsurv + __builtin_amdgcn_mbcnt_hi((uint32_t)(ball >> 32),
-                                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)ball, 0));
-                  if (h && slot < a.survCap)
-                     region[slot] = ((uint64_t)(uint32_t)(acc[v] + cs) << 32) | ((uint32_t)irel << 16) |
-                                    (uint32_t)jrel;
-                  nsurv += __builtin_popcountll(ball); // counts past survCap: region overflowed
+                  const int mq = max(max(acc[4 * q], acc[4 * q + 1]), max(acc[4 * q + 2], acc[4 * q + 3]));
+                  if (!__any(mq > lim)) continue;
+#pragma unroll
+                  for (int e = 0; e < 4; ++e)
+                  {
+                     const int v = 4 * q + e;
+                     const bool h = acc[v] > lim;
+                     const uint64_t ball = __ballot(h);
+                     if (ball == 0) continue;
+                     const int irel = 32 * b + 8 * q + 4 * half + e;
+                     const int slot = nsurv + __builtin_amdgcn_mbcnt_hi((uint32_t)(ball >> 32),
+                                                                        __builtin_amdgcn_mbcnt_lo((uint32_t)ball, 0));
+                     if (h && slot < a.survCap)
+                        region[slot] = ((uint64_t)(uint32_t)(acc[v] + cs) << 32) | ((uint32_t)irel << 16) |
+                                       (uint32_t)jrel;
+                     nsurv += __builtin_popcountll(ball); // counts past survCap: region overflowed
+                  }
                }
             }
          }

@@ -430,7 +430,7 @@ __global__ void k_flag(int n, int isRight, const uint64_t *__restrict__ top1, co
 }
 
 // Gathers rows idx[0..n) of an int8 pool and its row sums into a compact
-// pool of nPad rows (rows >= n zero).
+// pool of nPad rows (rows >= n zero, with the padding row sum).
 __global__ void k_gather_rows(int n, int nPad, const int *__restrict__ idx, const int8_t *__restrict__ src,
                               const int32_t *__restrict__ srcSum, int8_t *__restrict__ dst, int32_t *__restrict__ dstSum)
 {
@@ -438,7 +438,7 @@ __global__ void k_gather_rows(int n, int nPad, const int *__restrict__ idx, cons
    if (t >= (int64_t)nPad * (kDim / 16)) return;
    const int row = (int)(t / (kDim / 16)), part = (int)(t % (kDim / 16));
    uint4 v = make_uint4(0, 0, 0, 0);
-   int32_t s = 0;
+   int32_t s = kPadRowSum;
    if (row < n)
    {
       const int r = idx[row];

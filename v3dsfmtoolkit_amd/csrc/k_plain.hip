@@ -258,7 +258,7 @@ __global__ void k_prep_u8(const uint8_t *__restrict__ src, int n, int nPad, int8
    {
 #pragma unroll
       for (int q = 0; q < kDim / 16; ++q) d[q] = make_uint4(0x80808080u, 0x80808080u, 0x80808080u, 0x80808080u);
-      sum[row] = 0;
+      sum[row] = kPadRowSum;
    }
 }
 
@@ -296,7 +296,7 @@ __global__ void k_prep_f32(const float *__restrict__ src, int n, int nPad, int8_
    else
    {
       for (int q = 0; q < kDim; ++q) d[q] = (int8_t)-128;
-      sum[row] = 0;
+      sum[row] = kPadRowSum;
    }
 }
 

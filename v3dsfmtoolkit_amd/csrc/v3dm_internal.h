@@ -8,6 +8,11 @@ namespace v3dm {
 
 constexpr int kDim = 128;          // descriptor length
 constexpr int kRowPad = 128;       // descriptor pools are padded to multiples of 128 rows
+// Row-sum term of a padding row of an int8 pool.  In the scan a padding left
+// row's accumulator starts near -2^30 and a padding right row's threshold is
+// raised by 2^30, so neither passes the survivor test (scores of real rows are
+// below 2^24): the scan needs no per-step row or column bound checks.
+constexpr int32_t kPadRowSum = -(1 << 30);
 constexpr int kMaxMasks = 16;      // guided masks accumulated between two plain finds
 constexpr int kMaxLocalIndex = (1 << 21) - 1; // per-image feature limit of the packed keys
 constexpr uint32_t kEpochLimit = (1u << 20) - 2;
