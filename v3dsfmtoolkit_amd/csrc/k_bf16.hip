@@ -195,29 +195,35 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(2)))
             acc0 = mfma_bf16(A[0][k], Bc[k], acc0);
             acc1 = mfma_bf16(A[1][k], Bc[k], acc1);
          }
+         // NaN scores (padding rows and columns) pass neither test; fmaxf skips them
          const float m = fmaxf(fmax16(acc0), fmax16(acc1));
          if (__builtin_expect(__any(m > lim), 0))
          {
             const int jrel = (g * kGroupBF + s) * kStepCols + lrow;
-            const bool colOk = c0 + jrel < pd.nR;
 #pragma unroll
             for (int b = 0; b < 2; ++b)
             {
                const v16f &acc = b == 0 ? acc0 : acc1;
-               if (!__any(fmax16(acc) > lim)) continue;
 #pragma unroll
-               for (int v = 0; v < 16; ++v)
+               for (int q = 0; q < 4; ++q)
                {
-                  const int irel = 32 * b + 8 * (v >> 2) + 4 * half + (v & 3);
-                  const bool h = acc[v] > lim && colOk && l0 + irel < pd.nL;
-                  const uint64_t ball = __ballot(h);
-                  if (ball == 0) continue;
-                  const int slot = nsurv + __builtin_amdgcn_mbcnt_hi((uint32_t)(ball >> 32),
-                                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)ball, 0));
-                  if (h && slot < a.survCap)
-                     region[slot] = ((uint64_t)__float_as_uint(acc[v]) << 32) | ((uint32_t)irel << 16) |
-                                    (uint32_t)jrel;
-                  nsurv += __builtin_popcountll(ball);
+                  const float mq = fmaxf(fmaxf(acc[4 * q], acc[4 * q + 1]), fmaxf(acc[4 * q + 2], acc[4 * q + 3]));
+                  if (!__any(mq > lim)) continue;
+#pragma unroll
+                  for (int e = 0; e < 4; ++e)
+                  {
+                     const int v = 4 * q + e;
+                     const bool h = acc[v] > lim;
+                     const uint64_t ball = __ballot(h);
+                     if (ball == 0) continue;
+                     const int irel = 32 * b + 8 * q + 4 * half + e;
+                     const int slot = nsurv + __builtin_amdgcn_mbcnt_hi((uint32_t)(ball >> 32),
+                                                                        __builtin_amdgcn_mbcnt_lo((uint32_t)ball, 0));
+                     if (h && slot < a.survCap)
+                        region[slot] = ((uint64_t)__float_as_uint(acc[v]) << 32) | ((uint32_t)irel << 16) |
+                                       (uint32_t)jrel;
+                     nsurv += __builtin_popcountll(ball);
+                  }
                }
             }
          }
@@ -383,7 +389,9 @@ __global__ __launch_bounds__(256) void k_prep_bf16(const float *__restrict__ src
 #pragma unroll
    for (int i = 0; i < 4; ++i)
    {
-      w[i] = bf16_rne(v[2 * i]) | (bf16_rne(v[2 * i + 1]) << 16);
+      // padding rows are quiet NaNs: every score with them is NaN, which no
+      // survivor test passes (the scan needs no row or column bound checks)
+      w[i] = row < n ? bf16_rne(v[2 * i]) | (bf16_rne(v[2 * i + 1]) << 16) : 0x7FC07FC0u;
       n2 = fmaf(v[2 * i], v[2 * i], n2);
       n2 = fmaf(v[2 * i + 1], v[2 * i + 1], n2);
    }
