@@ -175,10 +175,11 @@ def run_c2(args, v):
     sel_avg_s = sel_ms / 1e3 / max(jobs, 1)
     if kernel == v.KERNEL_I8:
         achieved = pairs * 256 / scan_avg_s / 1e12
-        roof = {"bound": "mfma", "kernel": _scan_name(v, m.lastScanVariant()), "achieved": achieved,
+        scan_kernel = _scan_name(v, m.lastScanVariant(), m.lastScanInline() == 1)
+        roof = {"bound": "mfma", "kernel": scan_kernel, "achieved": achieved,
                 "peak": INT8_PEAK_TOPS,
                 "unit": "TFLOP/s", "frac": achieved / INT8_PEAK_TOPS,
-                "traffic": measured_traffic(_scan_name(v, m.lastScanVariant())) if n == 8192 else None,
+                "traffic": measured_traffic(scan_kernel) if n == 8192 else None,
                 "ops_per_pair": 256, "launch_avg_us": scan_avg_s * 1e6}
     else:
         achieved = pairs * 256 / scan_avg_s / 1e12
@@ -231,10 +232,11 @@ def run_c2(args, v):
     print(json.dumps(out), flush=True)
 
 
-def _scan_name(v, variant):
-    """Kernel name of a scan variant as rocprofv3 prints it, e.g. k_i8_scan<8, 512>."""
+def _scan_name(v, variant, inline=False):
+    """Kernel name of a scan variant as rocprofv3 prints it, e.g. k_i8_scan<8, 512, true>
+    (the last argument: the scan applies its survivors itself)."""
     w, _, tn = v.scan_variants()[variant]
-    return f"k_i8_scan<{w}, {tn}>"
+    return f"k_i8_scan<{w}, {tn}, {'true' if inline else 'false'}>"
 
 
 def _common(metric, value, unit, steps, warmup, wall, dtype, config, higher=True):

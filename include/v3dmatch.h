@@ -125,6 +125,10 @@ int v3dm_run_guided_homography_matching(v3dm_ctx *ctx, const double H[9], float 
 int v3dm_set_scan_variant(v3dm_ctx *ctx, int variant);
 /* The scan variant the last single-pair int8 job ran (-1 before any). */
 int v3dm_get_last_scan_variant(const v3dm_ctx *ctx, int *variant);
+/* Whether that job's scan applied its survivors itself (1: the whole grid was
+ * resident at once, no separate insertion pass) or left them to the insertion
+ * kernel (0); -1 before any job.  Diagnostic, no counterpart in the reference. */
+int v3dm_get_last_scan_inline(const v3dm_ctx *ctx, int *inlineInsert);
 
 /* Kernel selection (V3DM_KERNEL_*) and the kernel the last match call ran. */
 int v3dm_set_kernel(v3dm_ctx *ctx, int kernel);

@@ -137,6 +137,30 @@ def test_identical_sets_worst_case_survivors(m):
         assert run(m, L, L, *rule, kernel=v.KERNEL_I8) == oracle_pairs(L, L, *rule)
 
 
+@pytest.mark.parametrize("n,inline", [(4096, 1), (16384, 0)])
+def test_survivors_applied_in_the_scan_or_by_the_insertion_pass(gpu_available, n, inline):
+    """A grid that fits the device at once applies its survivors inside the scan
+    (regions, and exact recomputation of a block whose region overflows); a
+    larger grid leaves them to the insertion kernel.  Both give the oracle's
+    lists, with the filter on and with minRatio 0 (every score survives, every
+    region overflows)."""
+    import os
+    d = pair(3, 17, n, n)
+    mm = v.SIFTMatching(0)
+    mm.enableRoundingWarning(False)
+    try:
+        mm.allocate(n, n)
+        mm.setScanVariant(0)
+        for rule in [(0.8, 0.8), (0.0, 0.0)]:
+            say("survivor application", n, rule)
+            got = run(mm, d["L"], d["R"], *rule, kernel=v.KERNEL_I8)
+            assert mm.lastScanInline() == inline
+            tab = O.find_u8_blocked(d["L"], d["R"], *rule, threads=len(os.sched_getaffinity(0)))[2]
+            assert got == O.table_to_pairs(tab)
+    finally:
+        mm.close()
+
+
 def test_float_inputs_dispatch(m):
     say("float dispatch")
     d = pair(0, 8, 700, 900, f32=True)

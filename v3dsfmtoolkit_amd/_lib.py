@@ -87,6 +87,7 @@ _SIGNATURES = {
     "v3dm_copy_probe": (ctypes.c_int, [ctypes.c_int, ctypes.c_int64, ctypes.c_int]),
     "v3dm_set_scan_variant": (ctypes.c_int, [_ctx_p, ctypes.c_int]),
     "v3dm_get_last_scan_variant": (ctypes.c_int, [_ctx_p, _c_int_p]),
+    "v3dm_get_last_scan_inline": (ctypes.c_int, [_ctx_p, _c_int_p]),
     "v3dm_fetch_left_top2": (ctypes.c_int, [_ctx_p, ctypes.c_int, _f32_p, _f32_p, _i32_p]),
     "v3dm_mlp_param_count": (ctypes.c_int, []),
     "v3dm_mlp_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)]),

@@ -37,6 +37,7 @@
 // key, whose score is the reference's nextVal (a duplicate best counts).
 // Keys of older jobs carry an older epoch and read as empty, so no clearing
 // pass is needed between jobs.
+#include <atomic>
 #include <type_traits>
 
 #include <hip/hip_ext.h>
@@ -107,7 +108,7 @@ constexpr int kGroupBytes = kGroup * kStageBytes;  // 16 KiB
 // tiles blockIdx.x, then gridDim.x + atomicAdd(tileCounter) until the tiles
 // run out; the next index is requested when a tile starts, so its latency
 // hides behind the tile.
-template <int WAVES, int TILE_N>
+template <int WAVES, int TILE_N, bool INLINE_INSERT>
 __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(2))) void k_i8_scan(ScanArgs a)
 {
    constexpr int kScanWaves = WAVES;
@@ -278,7 +279,8 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(2)))
          if (__builtin_expect(__any(m > lim), 0))
          {
             // rare: append every surviving score of the wave to its region
-            // (plain stores, nothing waits on them); k_i8_insert applies them.
+            // (plain stores, nothing waits on them), applied when the tile
+            // is done (see below).
             // Padding rows and columns never pass (kPadRowSum), so there are
             // no bounds to check here; blocks of 4 registers are skipped
             // whole when none of their scores passes.
@@ -317,7 +319,44 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(2)))
       group(R1, g);
       if (g + 1 < ngroups) group(R0, g + 1);
    }
-   if (lane == 0) a.survCount[(size_t)tile * kScanWaves + wave] = nsurv;
+   // INLINE_INSERT (a grid that fits the device at once, so nothing waits
+   // for this wave's slot): the wave applies its own survivors -- the region's
+   // entries (its stores are complete before its lanes read them back), or,
+   // when the region overflowed (possible only when very many scores of one
+   // 64 x TILE_N block pass the filter, e.g. near-identical images or minRatio
+   // 0), every score of the block recomputed exactly from the descriptors.
+   // Otherwise k_i8_insert does the same from the region counts.
+   if constexpr (!INLINE_INSERT)
+   {
+      if (lane == 0) a.survCount[(size_t)tile * kScanWaves + wave] = nsurv;
+   }
+   else if (nsurv > 0)
+   {
+      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+      const uint32_t epoch = *a.epoch;
+      if (nsurv <= a.survCap)
+      {
+         for (int e = lane; e < nsurv; e += 64)
+         {
+            const uint64_t w = region[e];
+            survivor(a, pd, epoch, l0 + (int)((w >> 16) & 0xFFFF), c0 + (int)(w & 0xFFFF), (int32_t)(w >> 32));
+         }
+      }
+      else
+      {
+         const int nRows = min(kWaveRows, pd.nL - l0), nCols = min(kTileN, pd.nR - c0);
+         for (int e = lane; e < nRows * nCols; e += 64)
+         {
+            const int il = l0 + e / nCols, jr = c0 + e % nCols;
+            const uint32_t *x = reinterpret_cast<const uint32_t *>(a.L + (size_t)(pd.offL + il) * kDim);
+            const uint32_t *y = reinterpret_cast<const uint32_t *>(a.R + (size_t)(pd.offR + jr) * kDim);
+            uint32_t acc = 0;
+            for (int k = 0; k < kDim / 4; ++k) // stored bytes are u ^ 0x80
+               acc = __builtin_amdgcn_udot4(x[k] ^ 0x80808080u, y[k] ^ 0x80808080u, acc, false);
+            if ((int32_t)acc > a.thr) survivor(a, pd, epoch, il, jr, (int32_t)acc);
+         }
+      }
+   }
    if (!a.persistent) break;
    __syncthreads(); // every wave is done with this tile's LDS stages
    if (tid == 0) sNextTile = (int)nextReq;
@@ -462,16 +501,18 @@ __global__ void k_mfma_probe(int32_t *out)
 // ---- launchers ------------------------------------------------------------
 
 // Scan variants: (waves per workgroup, right rows per tile).
+constexpr int kMaxDevicesCached = 64;
+
 struct ScanVariant
 {
    int waves, tileN;
    const void *fn;
 };
 static const ScanVariant kVariants[] = {
-   {4, 512, reinterpret_cast<const void *>(&k_i8_scan<4, 512>)},
-   {4, 1024, reinterpret_cast<const void *>(&k_i8_scan<4, 1024>)},
-   {8, 512, reinterpret_cast<const void *>(&k_i8_scan<8, 512>)},
-   {4, 256, reinterpret_cast<const void *>(&k_i8_scan<4, 256>)},
+   {4, 512, reinterpret_cast<const void *>(&k_i8_scan<4, 512, false>)},
+   {4, 1024, reinterpret_cast<const void *>(&k_i8_scan<4, 1024, false>)},
+   {8, 512, reinterpret_cast<const void *>(&k_i8_scan<8, 512, false>)},
+   {4, 256, reinterpret_cast<const void *>(&k_i8_scan<4, 256, false>)},
 };
 
 int scan_variant_count() { return (int)(sizeof(kVariants) / sizeof(kVariants[0])); }
@@ -482,12 +523,22 @@ int scan_variant_tile_n(int v) { return kVariants[v].tileN; }
 // Workgroups of variant v resident on the whole device (for persistent launches).
 int scan_variant_resident(int v, int device)
 {
+   // asked on every job: remembered per device and variant (0 = not yet known)
+   static std::atomic<int> known[kMaxDevicesCached][4];
+   const bool cacheable = device >= 0 && device < kMaxDevicesCached && v >= 0 && v < 4;
+   if (cacheable)
+   {
+      const int k = known[device][v].load(std::memory_order_relaxed);
+      if (k > 0) return k;
+   }
    int perCU = 0, cus = 0;
    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kVariants[v].fn, kVariants[v].waves * 64, 0) !=
        hipSuccess)
       perCU = 1;
    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) cus = 1;
-   return (perCU > 0 ? perCU : 1) * (cus > 0 ? cus : 1);
+   const int r = (perCU > 0 ? perCU : 1) * (cus > 0 ? cus : 1);
+   if (cacheable) known[device][v].store(r, std::memory_order_relaxed);
+   return r;
 }
 
 hipError_t launch_i8_scan(int variant, int grid, const ScanArgs &a, hipStream_t st, hipEvent_t start,
@@ -495,12 +546,17 @@ hipError_t launch_i8_scan(int variant, int grid, const ScanArgs &a, hipStream_t 
 {
    if (a.nTiles <= 0) return hipSuccess;
    // start / stop (may be null) take the timestamps of this dispatch itself
-   switch (variant)
+   const int k = 2 * variant + (a.inlineInsert ? 1 : 0);
+   switch (k)
    {
-   case 0: hipExtLaunchKernelGGL((k_i8_scan<4, 512>), dim3(grid), dim3(256), 0, st, start, stop, 0, a); break;
-   case 1: hipExtLaunchKernelGGL((k_i8_scan<4, 1024>), dim3(grid), dim3(256), 0, st, start, stop, 0, a); break;
-   case 2: hipExtLaunchKernelGGL((k_i8_scan<8, 512>), dim3(grid), dim3(512), 0, st, start, stop, 0, a); break;
-   case 3: hipExtLaunchKernelGGL((k_i8_scan<4, 256>), dim3(grid), dim3(256), 0, st, start, stop, 0, a); break;
+   case 0: hipExtLaunchKernelGGL((k_i8_scan<4, 512, false>), dim3(grid), dim3(256), 0, st, start, stop, 0, a); break;
+   case 1: hipExtLaunchKernelGGL((k_i8_scan<4, 512, true>), dim3(grid), dim3(256), 0, st, start, stop, 0, a); break;
+   case 2: hipExtLaunchKernelGGL((k_i8_scan<4, 1024, false>), dim3(grid), dim3(256), 0, st, start, stop, 0, a); break;
+   case 3: hipExtLaunchKernelGGL((k_i8_scan<4, 1024, true>), dim3(grid), dim3(256), 0, st, start, stop, 0, a); break;
+   case 4: hipExtLaunchKernelGGL((k_i8_scan<8, 512, false>), dim3(grid), dim3(512), 0, st, start, stop, 0, a); break;
+   case 5: hipExtLaunchKernelGGL((k_i8_scan<8, 512, true>), dim3(grid), dim3(512), 0, st, start, stop, 0, a); break;
+   case 6: hipExtLaunchKernelGGL((k_i8_scan<4, 256, false>), dim3(grid), dim3(256), 0, st, start, stop, 0, a); break;
+   case 7: hipExtLaunchKernelGGL((k_i8_scan<4, 256, true>), dim3(grid), dim3(256), 0, st, start, stop, 0, a); break;
    default: return hipErrorInvalidValue;
    }
    return hipGetLastError();

@@ -306,6 +306,9 @@ int enqueueI8Job(v3dm_ctx *c, int v, float minScore, float minRatio, bool marks)
    a.surv = c->dSurv;
    a.survCount = c->dSurvCount;
    a.survCap = c->survCap;
+   // a grid the device holds at once applies its survivors in the scan; a
+   // larger one leaves them to k_i8_insert (a wave's tail would hold its slot)
+   a.inlineInsert = a.nTiles <= scan_variant_resident(v, c->device) ? 1 : 0;
    a.waves = scan_variant_waves(v);
    a.tileM = scan_variant_tile_m(v);
    a.tileN = scan_variant_tile_n(v);
@@ -319,7 +322,7 @@ int enqueueI8Job(v3dm_ctx *c, int v, float minScore, float minRatio, bool marks)
       HIPCHK(c, hipEventRecord(e1, c->stream));
    }
    HIPCHK(c, launch_i8_scan(v, a.nTiles, a, c->stream, e0, e1));
-   HIPCHK(c, launch_i8_insert(a, c->stream));
+   if (!a.inlineInsert) HIPCHK(c, launch_i8_insert(a, c->stream));
    FinalizeArgs f{};
    f.pairs = c->dPair;
    f.nPairs = 1;
@@ -338,6 +341,7 @@ int enqueueI8Job(v3dm_ctx *c, int v, float minScore, float minRatio, bool marks)
    if (marks && (rc = profMark(c, 2))) return rc;
    c->hostEpoch += 1;
    c->lastVariant = v;
+   c->lastInline = a.inlineInsert;
    return V3DM_OK;
 }
 
@@ -832,6 +836,13 @@ int v3dm_get_last_scan_variant(const v3dm_ctx *c, int *variant)
 {
    if (!c || !variant) return V3DM_ERR_ARG;
    *variant = c->lastVariant;
+   return V3DM_OK;
+}
+
+int v3dm_get_last_scan_inline(const v3dm_ctx *c, int *inlineInsert)
+{
+   if (!c || !inlineInsert) return V3DM_ERR_ARG;
+   *inlineInsert = c->lastInline;
    return V3DM_OK;
 }
 

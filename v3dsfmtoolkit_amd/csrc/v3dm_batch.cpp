@@ -228,6 +228,7 @@ void fillArgs(v3dm_ctx *c, BatchState *b, float minScore, float minRatio, ScanAr
    a.tilePair = b->dTilePair;
    const int resident = scan_variant_resident(v, c->device);
    a.persistent = b->nTiles > resident ? 1 : 0;
+   a.inlineInsert = a.persistent ? 0 : 1; // the whole grid resident: survivors applied in the scan
    grid = a.persistent ? resident : b->nTiles;
    f = FinalizeArgs{};
    f.pairs = b->dPairs;
@@ -308,8 +309,8 @@ int enqueueKernels(v3dm_ctx *c, BatchState *b, float minScore, float minRatio, b
    int rc;
    if (marks && (rc = profMark(c, 0))) return rc;
    HIPCHK(c, launch_i8_scan(b->variant, grid, a, c->stream));
+   if (!a.inlineInsert) HIPCHK(c, launch_i8_insert(a, c->stream));
    if (marks && (rc = profMark(c, 1))) return rc;
-   HIPCHK(c, launch_i8_insert(a, c->stream));
    HIPCHK(c, launch_i8_finalize(f, c->stream));
    HIPCHK(c, launch_advance_epoch(b->dCounters, b->dCounters + 2, c->stream));
    if (marks && (rc = profMark(c, 2))) return rc;

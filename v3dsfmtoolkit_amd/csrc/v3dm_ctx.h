@@ -21,6 +21,7 @@ int scan_variant_waves(int v);
 int scan_variant_tile_m(int v);
 int scan_variant_tile_n(int v);
 int scan_variant_resident(int v, int device);
+hipError_t launch_i8_insert(const ScanArgs &a, hipStream_t st);
 hipError_t launch_i8_finalize(const FinalizeArgs &a, hipStream_t st);
 hipError_t launch_bf16_scan(int variant, int grid, const ScanArgsF &a, hipStream_t st, hipEvent_t start = nullptr,
                             hipEvent_t stop = nullptr);
@@ -29,7 +30,6 @@ hipError_t launch_f32_rescore(const ScanArgsF &a, hipStream_t st);
 hipError_t launch_f32_finalize(const FinalizeArgs &a, hipStream_t st);
 hipError_t launch_prep_bf16(const float *src, int n, int nPad, uint16_t *dst, uint32_t *norm2Max, hipStream_t st,
                             bool resetNorm = true);
-hipError_t launch_i8_insert(const ScanArgs &a, hipStream_t st);
 hipError_t launch_advance_epoch(uint32_t *epoch, uint32_t *tileCounter, hipStream_t st);
 hipError_t launch_clear_keys(uint64_t *keys, int64_t n, uint32_t *epoch, uint32_t *done, hipStream_t st);
 hipError_t launch_mfma_probe(int32_t *out, hipStream_t st);
@@ -179,6 +179,7 @@ struct v3dm_ctx
    int survCap = 64;
    int scanVariant = -1;                                 // single-pair int8 scan variant, -1: measured
    int lastVariant = -1;                                 // variant of the last single-pair int8 job
+   int lastInline = -1;                                  // its scan applied the survivors itself (1) or not (0)
    std::vector<std::pair<uint64_t, int>> tunedI8;       // shape bucket -> fastest variant
 
    // float path

@@ -91,8 +91,9 @@ struct ScanArgs
    const Mask *masks;           // guided masks (single-pair jobs only)
    int nMasks;
    uint64_t *surv;              // survivor regions, survCap entries per (tile, wave)
-   int32_t *survCount;          // entries written per region (may exceed survCap)
+   int32_t *survCount;          // entries written per region (may exceed survCap); !inlineInsert only
    int survCap;
+   int inlineInsert;            // 1: each wave applies its region when its tile ends (no k_i8_insert)
    int waves, tileM, tileN;     // tile geometry of the launched scan variant
    int persistent;              // 1: resident workgroups pull tiles from *tileCounter
    uint32_t *tileCounter;       // reset to 0 by the finalize kernel

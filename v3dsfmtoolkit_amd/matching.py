@@ -84,6 +84,13 @@ class SIFTMatching:
         check(self._lib.v3dm_get_last_scan_variant(self._ctx, ctypes.byref(k)), self._ctx)
         return k.value
 
+    def lastScanInline(self) -> int:
+        """1 if the last single-pair int8 job's scan applied its survivors itself, 0 if
+        the insertion kernel did, -1 before any job."""
+        k = ctypes.c_int(-1)
+        check(self._lib.v3dm_get_last_scan_inline(self._ctx, ctypes.byref(k)), self._ctx)
+        return k.value
+
     def lastKernel(self) -> int:
         k = ctypes.c_int(0)
         check(self._lib.v3dm_get_last_kernel(self._ctx, ctypes.byref(k)), self._ctx)
