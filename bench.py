@@ -180,7 +180,12 @@ def run_c2(args, v):
                 "peak": INT8_PEAK_TOPS,
                 "unit": "TFLOP/s", "frac": achieved / INT8_PEAK_TOPS,
                 "traffic": measured_traffic(scan_kernel) if n == 8192 else None,
-                "ops_per_pair": 256, "launch_avg_us": scan_avg_s * 1e6}
+                "ops_per_pair": 256, "launch_avg_us": scan_avg_s * 1e6,
+                # the same work over the whole step (every kernel of the job, launch gaps included)
+                "job_frac": pairs * 256 / wall / 1e12 / INT8_PEAK_TOPS}
+        if m.lastScanInline() == 1:
+            roof["note"] = ("this grid fits the device at once, so the scan kernel also applies its survivors "
+                            "(top-2 insertion); no separate insertion kernel runs")
     else:
         achieved = pairs * 256 / scan_avg_s / 1e12
         roof = {"bound": "mfma", "kernel": "k_sgemm_seq", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
