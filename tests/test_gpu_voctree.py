@@ -222,3 +222,42 @@ def test_cpp_dropin_traversal_matches_oracle(big_tree, tmp_path):
     assert "nMaxFeatures should be a multiple of 32" in r.stderr
     leaves = np.fromfile(tmp_path / "leaves.bin", np.int32)
     np.testing.assert_array_equal(leaves, OV.quantise(3, 50, tree, keys) + 2550)
+
+
+def test_cpp_dropin_voctree_classes(big_tree, tmp_path):
+    """V3D::VocabularyTree<3, 50, 128> and V3D::LeafDocuments from
+    include/Base/v3d_voctree.h (tests/cpp/voctree_dropin_check.cpp, built with
+    fused multiply-adds available and contraction allowed): getVisualWord on
+    the host and getVisualWords on the GPU both give the oracle's words; the
+    driver's per-view binomial (and L1) scores against the earlier views equal
+    the inverted-file oracle's; a word outside the tree throws."""
+    import os
+    import subprocess
+    tree, _ = big_tree
+    keys = keys_like_sift(56, 3000)
+    tree.tofile(tmp_path / "tree.bin")
+    keys.tofile(tmp_path / "keys.bin")
+    exe = os.path.join(os.path.dirname(__file__), "cpp", "voctree_dropin_check")
+    per_view, n_best, p_match = 500, 3, np.float32(0.1)
+    say("voctree_dropin_check")
+    r = subprocess.run([exe, str(tmp_path / "tree.bin"), str(tmp_path / "keys.bin"), str(len(keys)), str(per_view),
+                        "0.1", str(n_best), str(tmp_path / "host.bin"), str(tmp_path / "gpu.bin"),
+                        str(tmp_path / "cand.txt")], capture_output=True, text=True, timeout=100)
+    assert r.returncode == 0, (r.returncode, r.stderr)
+    want = OV.quantise(3, 50, tree, keys)
+    np.testing.assert_array_equal(np.fromfile(tmp_path / "host.bin", np.int32), want)
+    np.testing.assert_array_equal(np.fromfile(tmp_path / "gpu.bin", np.int32), want)
+    docs = OV.LeafDocuments(125000)
+    expect = []
+    for view in range(len(keys) // per_view):
+        w = [int(x) for x in want[view * per_view:(view + 1) * per_view]]
+        expect += [f"{view} {d} {np.float32(sc)!r}" for sc, d in docs.binomial_scores(float(p_match), w, n_best)]
+        expect += [f"L1 {view} {d} {np.float32(sc)!r}" for sc, d in docs.l1_scores(w, n_best)]
+        docs.insert(w)
+    got = []
+    for line in (tmp_path / "cand.txt").read_text().splitlines():
+        f = line.split()
+        got.append(" ".join(f[:-1] + [repr(np.float32(float(f[-1])))]))
+    assert got == expect
+    # view v has v earlier documents to score
+    assert sum(1 for line in got if not line.startswith("L1")) == sum(min(v, n_best) for v in range(len(keys) // per_view))
