@@ -1,0 +1,45 @@
+"""The reference's own callers compile against the drop-in headers.
+
+SfM/match_sequence/src/match_sequence.cpp and
+SfM/match_using_voctree/src/match_using_voctree.cpp, with only the three cuBLAS
+lines deleted (the integration guide's step 3), are syntax-checked by g++ with
+include/ ahead of V3D/include, so include/CUDA/v3d_cudamatching.h (SIFT_Matching)
+and include/Base/v3d_voctree.h (VocabularyTree, LeafDocuments) replace the
+reference's headers and every use the drivers make of them type-checks.  The
+sources are read from the reference tree at test time into a temporary
+directory (nothing of them is kept here); the test is skipped where the
+reference tree is absent.  -fpermissive and V3DLIB_ENABLE_SQLITE3 are the
+reference's own build assumptions (its serialization header predates two-phase
+name lookup; its storage header forward-declares the SQLite types).
+"""
+import os
+import subprocess
+
+import pytest
+
+REF = "/root/reference"
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVERS = ["match_sequence", "match_using_voctree"]
+
+
+@pytest.mark.skipif(not os.path.isdir(os.path.join(REF, "SfM")), reason="reference tree not present")
+@pytest.mark.parametrize("name", DRIVERS)
+def test_reference_driver_compiles_against_dropin_headers(name, tmp_path):
+    src = os.path.join(REF, "SfM", name, "src", name + ".cpp")
+    with open(src, errors="replace") as f:
+        lines = f.readlines()
+    kept = [line for line in lines if "cublas" not in line]
+    assert len(lines) - len(kept) == 3  # the include, cublasInit(), cublasShutdown()
+    (tmp_path / (name + ".cpp")).write_text("".join(kept))
+    cmd = ["g++", "-std=c++17", "-fsyntax-only", "-w", "-fpermissive", "-DV3DLIB_ENABLE_SQLITE3", "-H",
+           "-I", os.path.join(ROOT, "include"), "-I", os.path.join(REF, "V3D", "include"),
+           "-I", os.path.join(REF, "V3D"), "-I", os.path.join(REF, "SfM", "reconstruction", "include"),
+           name + ".cpp"]
+    r = subprocess.run(cmd, cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-3000:]
+    included = [line.split()[-1] for line in r.stderr.splitlines() if line.startswith(".")]
+    ours = os.path.join(ROOT, "include")
+    assert os.path.join(ours, "CUDA", "v3d_cudamatching.h") in included
+    if name == "match_using_voctree":
+        assert os.path.join(ours, "Base", "v3d_voctree.h") in included
+    assert not any(p.startswith(REF) and p.endswith(("v3d_cudamatching.h", "v3d_voctree.h")) for p in included)
