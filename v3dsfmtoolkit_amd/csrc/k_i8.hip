@@ -238,7 +238,12 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(2)))
       }
    };
    v16i p0, p1; // scores of the step in flight
-   auto group = [&](auto slotc, int g) {
+   // `lastc` says at compile time whether g is the tile's last group: only
+   // its last step issues no next-step MFMAs, so the accumulator sets of the
+   // other groups never merge with a not-updated set (which made the
+   // compiler copy one set into the other at every group boundary).
+   auto group = [&](auto slotc, int g, auto lastc) {
+      constexpr bool kLast = decltype(lastc)::value;
       // ring slot `slotc` holds group g+1: stage it, then refill with group g+3
       store(slotc, (g + 1) % kThirds);
       fetch(slotc, g + 3);
@@ -259,7 +264,7 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(2)))
          // Fragments of the next step; after the barrier of group g the stage
          // of group g+1 is complete too (written before that barrier).
          v4i Bn[4];
-         const bool more = s + 1 < kGroup || g + 1 < ngroups;
+         const bool more = s + 1 < kGroup || !kLast; // constant once unrolled
          if (s + 1 < kGroup)
          {
 #pragma unroll
@@ -285,6 +290,10 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(2)))
             // no bounds to check here; blocks of 4 registers are skipped
             // whole when none of their scores passes.
             const int jrel = (g * kGroup + s) * kStepCols + lrow;
+            // recomputed here on every visit: hoisted out of the loop, the 32
+            // entry constants below would be spilled to scratch
+            int half4 = 4 * half;
+            asm volatile("" : "+v"(half4));
 #pragma unroll
             for (int b = 0; b < 2; ++b)
             {
@@ -301,7 +310,7 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(2)))
                      const bool h = acc[v] > lim;
                      const uint64_t ball = __ballot(h);
                      if (ball == 0) continue;
-                     const int irel = 32 * b + 8 * q + 4 * half + e;
+                     const int irel = 32 * b + 8 * q + half4 + e;
                      const int slot = nsurv + __builtin_amdgcn_mbcnt_hi((uint32_t)(ball >> 32),
                                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)ball, 0));
                      if (h && slot < a.survCap)
@@ -314,11 +323,21 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(2)))
          }
       }
    };
-   for (int g = 0; g < ngroups; g += 2)
+   const std::false_type kMore;
+   const std::true_type kEnd;
+   int g = 0;
+   for (; g + 2 < ngroups; g += 2)
    {
-      group(R1, g);
-      if (g + 1 < ngroups) group(R0, g + 1);
+      group(R1, g, kMore);
+      group(R0, g + 1, kMore);
    }
+   if (g + 1 < ngroups)
+   {
+      group(R1, g, kMore);
+      group(R0, g + 1, kEnd);
+   }
+   else if (g < ngroups)
+      group(R1, g, kEnd);
    // INLINE_INSERT (a grid that fits the device at once, so nothing waits
    // for this wave's slot): the wave applies its own survivors -- the region's
    // entries (its stores are complete before its lanes read them back), or,
