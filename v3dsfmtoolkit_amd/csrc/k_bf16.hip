@@ -160,7 +160,12 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(2)))
 
    const v16f zero = {};
    v4i Bc[kFragsBF];
-   auto group = [&](auto slotc, int g) {
+   // `lastc` says at compile time whether g is the tile's last group, so the
+   // next group's fragments are read, and renamed into Bc, unconditionally in
+   // every other group (a run-time test there made the compiler copy all the
+   // fragments at every group boundary).
+   auto group = [&](auto slotc, int g, auto lastc) {
+      constexpr bool kLast = decltype(lastc)::value;
       store(slotc, (g + 1) % kThirds);
       fetch(slotc, g + 3);
       lds_barrier();
@@ -181,7 +186,7 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(2)))
             for (int k = 0; k < kFragsBF; ++k)
                Bn[k] = *reinterpret_cast<const v4i *>(sb + (s + 1) * kStageBytesBF + fragoff[k]);
          }
-         else if (g + 1 < ngroups)
+         else if (!kLast)
          {
             const int8_t *sn = sB[(g + 1) % kThirds];
 #pragma unroll
@@ -200,6 +205,10 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(2)))
          if (__builtin_expect(__any(m > lim), 0))
          {
             const int jrel = (g * kGroupBF + s) * kStepCols + lrow;
+            // recomputed on every visit, so the entry constants below are not
+            // hoisted out of the loop (and spilled)
+            int half4 = 4 * half;
+            asm volatile("" : "+v"(half4));
 #pragma unroll
             for (int b = 0; b < 2; ++b)
             {
@@ -216,7 +225,7 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(2)))
                      const bool h = acc[v] > lim;
                      const uint64_t ball = __ballot(h);
                      if (ball == 0) continue;
-                     const int irel = 32 * b + 8 * q + 4 * half + e;
+                     const int irel = 32 * b + 8 * q + half4 + e;
                      const int slot = nsurv + __builtin_amdgcn_mbcnt_hi((uint32_t)(ball >> 32),
                                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)ball, 0));
                      if (h && slot < a.survCap)
@@ -227,18 +236,28 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(2)))
                }
             }
          }
-         if (s + 1 < kGroupBF || g + 1 < ngroups)
+         if (s + 1 < kGroupBF || !kLast)
          {
 #pragma unroll
             for (int k = 0; k < kFragsBF; ++k) Bc[k] = Bn[k];
          }
       }
    };
-   for (int g = 0; g < ngroups; g += 2)
+   const std::false_type kMore;
+   const std::true_type kEnd;
+   int g = 0;
+   for (; g + 2 < ngroups; g += 2)
    {
-      group(R1, g);
-      if (g + 1 < ngroups) group(R0, g + 1);
+      group(R1, g, kMore);
+      group(R0, g + 1, kMore);
    }
+   if (g + 1 < ngroups)
+   {
+      group(R1, g, kMore);
+      group(R0, g + 1, kEnd);
+   }
+   else if (g < ngroups)
+      group(R1, g, kEnd);
    if (lane == 0) a.survCount[(size_t)tile * WAVES + wave] = nsurv;
    if (!a.persistent) break;
    __syncthreads();
