@@ -206,36 +206,39 @@ __global__ __launch_bounds__(64) void k_vt_level_mfma(const float *__restrict__ 
             for (int nb = 0; nb < NB; ++nb) n[j][nb] = grp[(kSeg * seg + j) * 64 + 32 * nb];
          }
       };
-      loadSeg(0, kr, nc);
-#pragma unroll 1
-      for (int seg = 0; seg < 16 / kSeg; ++seg) // elements 64 h + 4 kSeg seg ..; the next segment in flight
-      {
-         float4 kn[kSeg], nn[kSeg][NB];
-         if (seg + 1 < 16 / kSeg) loadSeg(seg + 1, kn, nn);
+      auto computeSeg = [&](const float4(&k)[kSeg], const float4(&n)[kSeg][NB]) {
 #pragma unroll
          for (int j = 0; j < kSeg; ++j)
 #pragma unroll
             for (int q = 0; q < 4; ++q)
             {
-               const float kv = q == 0 ? kr[j].x : q == 1 ? kr[j].y : q == 2 ? kr[j].z : kr[j].w;
+               const float kv = q == 0 ? k[j].x : q == 1 ? k[j].y : q == 2 ? k[j].z : k[j].w;
                ss = fmaf(kv, kv, ss);
 #pragma unroll
                for (int nb = 0; nb < NB; ++nb)
                {
-                  const float nv = q == 0 ? nc[j][nb].x : q == 1 ? nc[j][nb].y : q == 2 ? nc[j][nb].z : nc[j][nb].w;
+                  const float nv = q == 0 ? n[j][nb].x : q == 1 ? n[j][nb].y : q == 2 ? n[j][nb].z : n[j][nb].w;
                   acc[nb] = __builtin_amdgcn_mfma_f32_32x32x2f32(nv, kv, acc[nb], 0, 0, 0);
                }
             }
-         if (seg + 1 < 16 / kSeg)
-         {
-#pragma unroll
-            for (int j = 0; j < kSeg; ++j)
-            {
-               kr[j] = kn[j];
-#pragma unroll
-               for (int nb = 0; nb < NB; ++nb) nc[j][nb] = nn[j][nb];
-            }
-         }
+      };
+      // Segments 64 h + 4 kSeg seg .., two at a time through two register sets
+      // with the next segment always in flight; the load into the first set
+      // targets registers already consumed, so no set is ever copied (a rolled
+      // loop's run-time "is there a next segment" test made the compiler copy
+      // every prefetched register on each iteration).
+      constexpr int kSegs = 16 / kSeg;
+      static_assert(kSegs % 2 == 0, "segments go in pairs");
+      float4 kn[kSeg], nn[kSeg][NB];
+      loadSeg(0, kr, nc);
+#pragma unroll 1
+      for (int seg = 0; seg < kSegs; seg += 2)
+      {
+         loadSeg(seg + 1, kn, nn);
+         computeSeg(kr, nc);
+         asm volatile("" : "+v"(ss)); // keeps the sum of squares of kr here, not sunk past the load below
+         if (seg + 2 < kSegs) loadSeg(seg + 2, kr, nc);
+         computeSeg(kn, nn);
       }
 
       // candidate mask of key 32 kb + lj (both lane halves end with the same mask):
