@@ -79,7 +79,7 @@ __device__ inline v16f emb_block(const v4i (&A)[4], const v4i (&Bq)[4])
 // unit p of a row is stored at unit p ^ (row & 7) (conflict-free fragment reads).
 constexpr int kSweepStageRows = 256;
 
-__global__ __launch_bounds__(256) void k_emb_sweep(int collect, int nQ, int nR, const uint16_t *__restrict__ EQ,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_emb_sweep(int collect, int nQ, int nR, const uint16_t *__restrict__ EQ,
                                                    const uint16_t *__restrict__ ER, int chunkRows,
                                                    float *__restrict__ gmax, const float *__restrict__ theta,
                                                    float *__restrict__ listS, int *__restrict__ listI,

@@ -293,7 +293,7 @@ __global__ void k_embed_weights(const float *__restrict__ Pr, uint16_t *__restri
    }
 }
 
-__global__ __launch_bounds__(256) void k_embed_mfma(int n, const uint8_t *__restrict__ U8,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_embed_mfma(int n, const uint8_t *__restrict__ U8,
                                                     const uint16_t *__restrict__ W1bf, const float *__restrict__ b1,
                                                     const float *__restrict__ W2T, const float *__restrict__ b2,
                                                     float *__restrict__ outF, uint16_t *__restrict__ outBF)

@@ -145,7 +145,7 @@ struct VtNearTie
 };
 
 template <int NB>
-__global__ __launch_bounds__(64) void k_vt_level_mfma(const float *__restrict__ keys,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_vt_level_mfma(const float *__restrict__ keys,
                                                      const float4 *__restrict__ treeT,
                                                      const float *__restrict__ nodeNorm,
                                                      const int32_t *__restrict__ order,
