@@ -295,8 +295,8 @@ __device__ inline void rescore_insert(const ScanArgsF &a, const PairDesc &pd, ui
    }
    const float s = dot_seq(a.Lf + (size_t)(pd.offL + il) * kDim, a.Rf + (size_t)(pd.offR + jr) * kDim);
    if (!(s > a.thr)) return;
-   top2_insert(a.top1L + pd.slotL + il, a.top2L + pd.slotL + il, pack_keyf(epoch, s, (uint32_t)jr));
-   top2_insert(a.top1R + pd.slotR + jr, a.top2R + pd.slotR + jr, pack_keyf(epoch, s, (uint32_t)il));
+   top2_insert_both(a.top1L + pd.slotL + il, a.top2L + pd.slotL + il, pack_keyf(epoch, s, (uint32_t)jr),
+                    a.top1R + pd.slotR + jr, a.top2R + pd.slotR + jr, pack_keyf(epoch, s, (uint32_t)il));
 }
 
 // One wave per (tile, wave) region: every candidate rescored in FP32 and

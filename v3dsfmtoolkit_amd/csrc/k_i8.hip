@@ -62,8 +62,8 @@ __device__ __forceinline__ void survivor(const ScanArgs &a, const PairDesc &pd, 
       const Mask &mk = a.masks[m];
       if (mask_hit(mk, mk.lpos[2 * il], mk.lpos[2 * il + 1], mk.rpos[2 * jr], mk.rpos[2 * jr + 1])) return;
    }
-   top2_insert(a.top1L + pd.slotL + il, a.top2L + pd.slotL + il, pack_key(epoch, (uint32_t)score, (uint32_t)jr));
-   top2_insert(a.top1R + pd.slotR + jr, a.top2R + pd.slotR + jr, pack_key(epoch, (uint32_t)score, (uint32_t)il));
+   top2_insert_both(a.top1L + pd.slotL + il, a.top2L + pd.slotL + il, pack_key(epoch, (uint32_t)score, (uint32_t)jr),
+                    a.top1R + pd.slotR + jr, a.top2R + pd.slotR + jr, pack_key(epoch, (uint32_t)score, (uint32_t)il));
 }
 
 __device__ inline int max16(const v16i &x)

@@ -84,6 +84,18 @@ __device__ inline void top2_insert(uint64_t *t1, uint64_t *t2, uint64_t key)
    if (old != key) atomicMax((unsigned long long *)t2, (unsigned long long)(old < key ? old : key));
 }
 
+// The same for a left and a right key at once: both top-1 atomics are issued
+// before either result is waited for (the two arrays are independent), so a
+// survivor costs one atomic round trip, not two.
+__device__ inline void top2_insert_both(uint64_t *t1a, uint64_t *t2a, uint64_t ka, uint64_t *t1b, uint64_t *t2b,
+                                        uint64_t kb)
+{
+   const unsigned long long oa = atomicMax((unsigned long long *)t1a, (unsigned long long)ka);
+   const unsigned long long ob = atomicMax((unsigned long long *)t1b, (unsigned long long)kb);
+   if (oa != ka) atomicMax((unsigned long long *)t2a, (unsigned long long)(oa < ka ? oa : ka));
+   if (ob != kb) atomicMax((unsigned long long *)t2b, (unsigned long long)(ob < kb ? ob : kb));
+}
+
 __device__ inline v4i ld16(const int8_t *p)
 {
    return *(__attribute__((address_space(1))) const v4i *)(p);
