@@ -306,21 +306,25 @@ __global__ __launch_bounds__(256) void k_f32_rescore(ScanArgsF a)
    const int lane = threadIdx.x & 63;
    const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
    if (r >= a.nTiles * a.waves) return;
-   const int cnt = a.survCount[r];
-   if (cnt == 0) return;
+   // the count, the tile's pair, the epoch and the region's first 64 entries
+   // (inside the region: survCap >= 64, checked on the host) are loaded before
+   // any of them is waited for; stale entries past the count are not used
+   const uint64_t *region = a.surv + (size_t)r * a.survCap;
+   const uint64_t w0 = region[lane];
    const int tile = r / a.waves, wave = r - tile * a.waves;
    const int p = pair_of_tile(a.tilePair, a.pairs, a.nPairs, tile);
+   const uint32_t epoch = *a.epoch;
+   const int cnt = a.survCount[r];
+   if (cnt == 0) return;
    const PairDesc pd = a.pairs[p];
    const int t = tile - pd.tileBase;
    const int tm = t / pd.tilesN, tn = t - tm * pd.tilesN;
    const int l0 = tm * a.tileM + wave * kWaveRows, c0 = tn * a.tileN;
-   const uint32_t epoch = *a.epoch;
    if (cnt <= a.survCap)
    {
-      const uint64_t *region = a.surv + (size_t)r * a.survCap;
       for (int e = lane; e < cnt; e += 64)
       {
-         const uint64_t w = region[e];
+         const uint64_t w = e == lane ? w0 : region[e];
          rescore_insert(a, pd, epoch, l0 + (int)((w >> 16) & 0xFFFF), c0 + (int)(w & 0xFFFF));
       }
       return;
@@ -464,6 +468,7 @@ hipError_t launch_f32_rescore(const ScanArgsF &a, hipStream_t st)
 {
    const int regions = a.nTiles * a.waves;
    if (regions <= 0) return hipSuccess;
+   if (a.survCap < 64) return hipErrorInvalidValue; // each wave reads its region's first 64 entries
    hipLaunchKernelGGL(k_f32_rescore, dim3((regions + 3) / 4), dim3(256), 0, st, a);
    return hipGetLastError();
 }
