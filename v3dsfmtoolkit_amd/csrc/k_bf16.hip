@@ -342,25 +342,27 @@ __device__ inline float keyf_value(uint64_t k, uint32_t epoch)
 __global__ __launch_bounds__(256) void k_f32_finalize(FinalizeArgs a)
 {
    const int blk = blockIdx.x;
-   const int p = pair_of_fin(a.finPair, a.pairs, a.nPairs, blk);
-   const PairDesc pd = a.pairs[p];
+   PairDesc pd = a.pair0;
+   if (!a.hasPair0) pd = a.pairs[pair_of_fin(a.finPair, a.pairs, a.nPairs, blk)];
    const int k = (blk - pd.finBase) * 256 + threadIdx.x;
    const uint32_t epoch = *a.epoch;
    if (k < pd.nL)
    {
       int out = -1;
       const uint64_t t1 = a.top1L[pd.slotL + k];
+      const uint64_t t2 = a.top2L[pd.slotL + k];
       if (keyf_epoch(t1) == epoch)
       {
-         const float best = keyf_score(t1);
          const uint32_t j = key_index(t1);
-         const float next = keyf_value(a.top2L[pd.slotL + k], epoch);
+         const uint64_t u1 = a.top1R[pd.slotR + j];
+         const uint64_t u2 = a.top2R[pd.slotR + j];
+         const float best = keyf_score(t1);
+         const float next = keyf_value(t2, epoch);
          if (rule_accept(best, next, a.minScore, a.minRatio))
          {
-            const uint64_t u1 = a.top1R[pd.slotR + j];
             if (keyf_epoch(u1) == epoch && key_index(u1) == (uint32_t)k)
             {
-               const float nextR = keyf_value(a.top2R[pd.slotR + j], epoch);
+               const float nextR = keyf_value(u2, epoch);
                if (rule_accept(keyf_score(u1), nextR, a.minScore, a.minRatio)) out = (int)j;
             }
          }

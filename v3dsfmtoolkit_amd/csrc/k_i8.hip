@@ -443,25 +443,29 @@ __device__ inline float key_value(uint64_t k, uint32_t epoch)
 __global__ __launch_bounds__(256) void k_i8_finalize(FinalizeArgs a)
 {
    const int blk = blockIdx.x;
-   const int p = pair_of_fin(a.finPair, a.pairs, a.nPairs, blk);
-   const PairDesc pd = a.pairs[p];
+   PairDesc pd = a.pair0;
+   if (!a.hasPair0) pd = a.pairs[pair_of_fin(a.finPair, a.pairs, a.nPairs, blk)];
    const int k = (blk - pd.finBase) * 256 + threadIdx.x;
    const uint32_t epoch = *a.epoch;
    if (k < pd.nL)
    {
+      // two dependent rounds of loads: the left's keys, then (for a key of
+      // this job, whose right index is valid) the chosen right's keys
       int out = -1;
       const uint64_t t1 = a.top1L[pd.slotL + k];
+      const uint64_t t2 = a.top2L[pd.slotL + k];
       if (key_epoch(t1) == epoch)
       {
-         const float best = key_value(t1, epoch);
          const uint32_t j = key_index(t1);
-         const float next = key_value(a.top2L[pd.slotL + k], epoch);
+         const uint64_t u1 = a.top1R[pd.slotR + j];
+         const uint64_t u2 = a.top2R[pd.slotR + j];
+         const float best = key_value(t1, epoch);
+         const float next = key_value(t2, epoch);
          if (rule_accept(best, next, a.minScore, a.minRatio))
          {
-            const uint64_t u1 = a.top1R[pd.slotR + j];
             if (key_epoch(u1) == epoch && key_index(u1) == (uint32_t)k)
             {
-               const float nextR = key_value(a.top2R[pd.slotR + j], epoch);
+               const float nextR = key_value(u2, epoch);
                if (rule_accept(key_value(u1, epoch), nextR, a.minScore, a.minRatio)) out = (int)j;
             }
          }

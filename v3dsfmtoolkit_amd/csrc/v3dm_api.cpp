@@ -337,6 +337,8 @@ int enqueueI8Job(v3dm_ctx *c, int v, float minScore, float minRatio, bool marks)
    f.minScore = minScore;
    f.minRatio = minRatio;
    f.matchOfLeft = c->dMatch;
+   f.pair0 = pd;
+   f.hasPair0 = 1;
    HIPCHK(c, launch_i8_finalize(f, c->stream));
    if (marks && (rc = profMark(c, 2))) return rc;
    c->hostEpoch += 1;
@@ -476,6 +478,8 @@ int enqueueBF16(v3dm_ctx *c, float minScore, float minRatio)
    f.minScore = minScore;
    f.minRatio = minRatio;
    f.matchOfLeft = c->dMatch;
+   f.pair0 = pd;
+   f.hasPair0 = 1;
    HIPCHK(c, launch_f32_finalize(f, c->stream));
    if ((rc = profMark(c, 2))) return rc;
    c->hostEpoch += 1;

@@ -137,6 +137,8 @@ struct FinalizeArgs
    float minScore, minRatio;
    int32_t *matchOfLeft; // per left slot: matched right index or -1
    const int32_t *finPair; // optional: index into pairs of every finalize block
+   PairDesc pair0;         // the job's one pair, by value (hasPair0): no dependent load of it
+   int hasPair0;
 };
 
 } // namespace v3dm

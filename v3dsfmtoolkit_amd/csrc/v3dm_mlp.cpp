@@ -617,6 +617,8 @@ int v3dm_learned_find(v3dm_ctx *c, v3dm_mlp *m, int topK, float margin, float mi
    f.minScore = minScore;
    f.minRatio = minRatio;
    f.matchOfLeft = c->dMatch;
+   f.pair0 = pd;
+   f.hasPair0 = 1;
    HIPCHK(c, launch_i8_finalize(f, c->stream));
    c->hostEpoch += 1;
    c->lastKernel = V3DM_KERNEL_I8;
