@@ -389,45 +389,68 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(2)))
 // A region whose count exceeds its capacity (possible only when very many
 // scores of one 64 x 512 block pass the filter, e.g. near-identical images)
 // is recomputed here from the descriptors, so the result stays exact.
+// One wave per tile, for all of the tile's regions (kMaxScanWaves at most):
+// every region's count and first 64 entries (inside the region: survCap >= 64,
+// checked on the host), the tile's pair and the epoch are loaded before any of
+// them is waited for, and the pair is looked up once for the tile; stale
+// entries past a count are not used.
+constexpr int kMaxScanWaves = 8;
 __global__ __launch_bounds__(256) void k_i8_insert(ScanArgs a)
 {
    const int lane = threadIdx.x & 63;
-   const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
-   if (r >= a.nTiles * a.waves) return;
-   // The count, the tile's pair, the epoch and the region's first 64 entries
-   // (inside the region: survCap >= 64, checked on the host) are all loaded
-   // before any of them is waited for; stale entries past the count are not used.
-   const uint64_t *region = a.surv + (size_t)r * a.survCap;
-   const uint64_t w0 = region[lane];
-   const int tile = r / a.waves, wave = r - tile * a.waves;
+   const int tile = blockIdx.x * 4 + (threadIdx.x >> 6);
+   if (tile >= a.nTiles) return;
+   const size_t r0 = (size_t)tile * a.waves;
+   int cnt[kMaxScanWaves];
+   uint64_t w0[kMaxScanWaves];
+#pragma unroll
+   for (int w = 0; w < kMaxScanWaves; ++w)
+   {
+      cnt[w] = 0;
+      w0[w] = 0;
+      if (w < a.waves)
+      {
+         cnt[w] = a.survCount[r0 + w];
+         w0[w] = a.surv[(r0 + w) * a.survCap + lane];
+      }
+   }
    const int p = pair_of_tile(a.tilePair, a.pairs, a.nPairs, tile);
    const uint32_t epoch = *a.epoch;
-   const int cnt = a.survCount[r];
-   if (cnt == 0) return;
+   int any = 0;
+#pragma unroll
+   for (int w = 0; w < kMaxScanWaves; ++w) any |= cnt[w];
+   if (any == 0) return;
    const PairDesc pd = a.pairs[p];
    const int t = tile - pd.tileBase;
    const int tm = t / pd.tilesN, tn = t - tm * pd.tilesN;
-   const int l0 = tm * a.tileM + wave * kWaveRows, c0 = tn * a.tileN;
-   if (cnt <= a.survCap)
+   const int c0 = tn * a.tileN;
+#pragma unroll
+   for (int w = 0; w < kMaxScanWaves; ++w)
    {
-      for (int e = lane; e < cnt; e += 64)
+      if (cnt[w] == 0) continue;
+      const int l0 = tm * a.tileM + w * kWaveRows;
+      const uint64_t *region = a.surv + (r0 + w) * a.survCap;
+      if (cnt[w] <= a.survCap)
       {
-         const uint64_t w = e == lane ? w0 : region[e];
-         survivor(a, pd, epoch, l0 + (int)((w >> 16) & 0xFFFF), c0 + (int)(w & 0xFFFF), (int32_t)(w >> 32));
+         for (int e = lane; e < cnt[w]; e += 64)
+         {
+            const uint64_t v = e == lane ? w0[w] : region[e];
+            survivor(a, pd, epoch, l0 + (int)((v >> 16) & 0xFFFF), c0 + (int)(v & 0xFFFF), (int32_t)(v >> 32));
+         }
+         continue;
       }
-      return;
-   }
-   // overflowed region: every (left, right) of the block, exact u8 dot products
-   const int nRows = min(kWaveRows, pd.nL - l0), nCols = min(a.tileN, pd.nR - c0);
-   for (int e = lane; e < nRows * nCols; e += 64)
-   {
-      const int il = l0 + e / nCols, jr = c0 + e % nCols;
-      const uint32_t *x = reinterpret_cast<const uint32_t *>(a.L + (size_t)(pd.offL + il) * kDim);
-      const uint32_t *y = reinterpret_cast<const uint32_t *>(a.R + (size_t)(pd.offR + jr) * kDim);
-      uint32_t acc = 0;
-      for (int k = 0; k < kDim / 4; ++k) // stored bytes are u ^ 0x80
-         acc = __builtin_amdgcn_udot4(x[k] ^ 0x80808080u, y[k] ^ 0x80808080u, acc, false);
-      if ((int32_t)acc > a.thr) survivor(a, pd, epoch, il, jr, (int32_t)acc);
+      // overflowed region: every (left, right) of the block, exact u8 dot products
+      const int nRows = min(kWaveRows, pd.nL - l0), nCols = min(a.tileN, pd.nR - c0);
+      for (int e = lane; e < nRows * nCols; e += 64)
+      {
+         const int il = l0 + e / nCols, jr = c0 + e % nCols;
+         const uint32_t *x = reinterpret_cast<const uint32_t *>(a.L + (size_t)(pd.offL + il) * kDim);
+         const uint32_t *y = reinterpret_cast<const uint32_t *>(a.R + (size_t)(pd.offR + jr) * kDim);
+         uint32_t acc = 0;
+         for (int k = 0; k < kDim / 4; ++k) // stored bytes are u ^ 0x80
+            acc = __builtin_amdgcn_udot4(x[k] ^ 0x80808080u, y[k] ^ 0x80808080u, acc, false);
+         if ((int32_t)acc > a.thr) survivor(a, pd, epoch, il, jr, (int32_t)acc);
+      }
    }
 }
 
@@ -592,9 +615,9 @@ hipError_t launch_i8_scan(int variant, int grid, const ScanArgs &a, hipStream_t 
 hipError_t launch_i8_insert(const ScanArgs &a, hipStream_t st)
 {
    if (a.nTiles <= 0) return hipSuccess;
-   if (a.survCap < 64) return hipErrorInvalidValue; // each wave reads its region's first 64 entries
-   const int regions = a.nTiles * a.waves;
-   hipLaunchKernelGGL(k_i8_insert, dim3((regions + 3) / 4), dim3(256), 0, st, a);
+   if (a.survCap < 64) return hipErrorInvalidValue; // each wave reads its regions' first 64 entries
+   if (a.waves < 1 || a.waves > kMaxScanWaves) return hipErrorInvalidValue;
+   hipLaunchKernelGGL(k_i8_insert, dim3((a.nTiles + 3) / 4), dim3(256), 0, st, a);
    return hipGetLastError();
 }
 
