@@ -389,26 +389,44 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(2)))
 // A region whose count exceeds its capacity (possible only when very many
 // scores of one 64 x 512 block pass the filter, e.g. near-identical images)
 // is recomputed here from the descriptors, so the result stays exact.
-// One wave per tile, for all of the tile's regions (kMaxScanWaves at most):
-// every region's count and first 64 entries (inside the region: survCap >= 64,
-// checked on the host), the tile's pair and the epoch are loaded before any of
-// them is waited for, and the pair is looked up once for the tile; stale
-// entries past a count are not used.
+// kPerTile: one wave per tile, for all of the tile's regions (kMaxScanWaves
+// at most; large grids, where the per-wave latency chains are what costs);
+// otherwise one wave per region (small grids, where more waves in flight
+// pay).  Every region's count and first 64 entries (inside the region:
+// survCap >= 64, checked on the host), the tile's pair and the epoch are
+// loaded before any of them is waited for, and the pair is looked up once per
+// wave; stale entries past a count are not used.
 constexpr int kMaxScanWaves = 8;
+template <bool kPerTile>
 __global__ __launch_bounds__(256) void k_i8_insert(ScanArgs a)
 {
+   constexpr int kSpan = kPerTile ? kMaxScanWaves : 1; // regions a wave may take
    const int lane = threadIdx.x & 63;
-   const int tile = blockIdx.x * 4 + (threadIdx.x >> 6);
-   if (tile >= a.nTiles) return;
-   const size_t r0 = (size_t)tile * a.waves;
-   int cnt[kMaxScanWaves];
-   uint64_t w0[kMaxScanWaves];
+   const int wid = blockIdx.x * 4 + (threadIdx.x >> 6);
+   int tile, wFirst, wCount;
+   if (kPerTile)
+   {
+      if (wid >= a.nTiles) return;
+      tile = wid;
+      wFirst = 0;
+      wCount = a.waves;
+   }
+   else
+   {
+      if (wid >= a.nTiles * a.waves) return;
+      tile = wid / a.waves;
+      wFirst = wid - tile * a.waves;
+      wCount = 1;
+   }
+   const size_t r0 = (size_t)tile * a.waves + wFirst;
+   int cnt[kSpan];
+   uint64_t w0[kSpan];
 #pragma unroll
-   for (int w = 0; w < kMaxScanWaves; ++w)
+   for (int w = 0; w < kSpan; ++w)
    {
       cnt[w] = 0;
       w0[w] = 0;
-      if (w < a.waves)
+      if (w < wCount)
       {
          cnt[w] = a.survCount[r0 + w];
          w0[w] = a.surv[(r0 + w) * a.survCap + lane];
@@ -418,17 +436,17 @@ __global__ __launch_bounds__(256) void k_i8_insert(ScanArgs a)
    const uint32_t epoch = *a.epoch;
    int any = 0;
 #pragma unroll
-   for (int w = 0; w < kMaxScanWaves; ++w) any |= cnt[w];
+   for (int w = 0; w < kSpan; ++w) any |= cnt[w];
    if (any == 0) return;
    const PairDesc pd = a.pairs[p];
    const int t = tile - pd.tileBase;
    const int tm = t / pd.tilesN, tn = t - tm * pd.tilesN;
    const int c0 = tn * a.tileN;
 #pragma unroll
-   for (int w = 0; w < kMaxScanWaves; ++w)
+   for (int w = 0; w < kSpan; ++w)
    {
       if (cnt[w] == 0) continue;
-      const int l0 = tm * a.tileM + w * kWaveRows;
+      const int l0 = tm * a.tileM + (wFirst + w) * kWaveRows;
       const uint64_t *region = a.surv + (r0 + w) * a.survCap;
       if (cnt[w] <= a.survCap)
       {
@@ -617,7 +635,11 @@ hipError_t launch_i8_insert(const ScanArgs &a, hipStream_t st)
    if (a.nTiles <= 0) return hipSuccess;
    if (a.survCap < 64) return hipErrorInvalidValue; // each wave reads its regions' first 64 entries
    if (a.waves < 1 || a.waves > kMaxScanWaves) return hipErrorInvalidValue;
-   hipLaunchKernelGGL(k_i8_insert, dim3((a.nTiles + 3) / 4), dim3(256), 0, st, a);
+   const int64_t regions = (int64_t)a.nTiles * a.waves;
+   if (regions > 32768) // 256-pair batches (36 k tiles): 127 -> 116 us per tile
+      hipLaunchKernelGGL(k_i8_insert<true>, dim3((a.nTiles + 3) / 4), dim3(256), 0, st, a);
+   else // 8192^2 with 256 x 256 tiles (1024 tiles): 4.0 us per region, 7.3 us per tile
+      hipLaunchKernelGGL(k_i8_insert<false>, dim3((unsigned)((regions + 3) / 4)), dim3(256), 0, st, a);
    return hipGetLastError();
 }
 
