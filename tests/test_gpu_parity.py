@@ -161,6 +161,34 @@ def test_survivors_applied_in_the_scan_or_by_the_insertion_pass(gpu_available, n
         mm.close()
 
 
+@pytest.mark.parametrize("shape,per_tile", [((4, 256, 256), True), ((4, 256, 512), False)])
+def test_insertion_pass_per_tile_and_per_region(gpu_available, shape, per_tile):
+    """The insertion kernel takes one wave per tile above 32768 survivor
+    regions and one wave per region below: a 32768 x 32768 pair has 65536
+    regions with 256 x 256 tiles and 32768 with 256 x 512 tiles.  Both give the
+    oracle's lists, with the filter on and with minRatio 0 (every region
+    overflows and is recomputed from the descriptors)."""
+    import os
+    variant = v.scan_variants().index(shape)
+    waves, rows, cols = shape
+    n = 32768
+    assert ((n // rows) * (n // cols) * waves > 32768) == per_tile
+    d = pair(3, 29, n, n)
+    mm = v.SIFTMatching(0)
+    mm.enableRoundingWarning(False)
+    try:
+        mm.allocate(n, n)
+        mm.setScanVariant(variant)
+        for rule in [(0.8, 0.8), (0.0, 0.0)]:
+            say("insertion pass", shape, rule)
+            got = run(mm, d["L"], d["R"], *rule, kernel=v.KERNEL_I8)
+            assert mm.lastScanInline() == 0
+            tab = O.find_u8_blocked(d["L"], d["R"], *rule, threads=len(os.sched_getaffinity(0)))[2]
+            assert got == O.table_to_pairs(tab)
+    finally:
+        mm.close()
+
+
 def test_float_inputs_dispatch(m):
     say("float dispatch")
     d = pair(0, 8, 700, 900, f32=True)
