@@ -130,6 +130,17 @@ int v3dm_get_last_scan_variant(const v3dm_ctx *ctx, int *variant);
  * kernel (0); -1 before any job.  Diagnostic, no counterpart in the reference. */
 int v3dm_get_last_scan_inline(const v3dm_ctx *ctx, int *inlineInsert);
 
+/* Survivor insertion pass of int8 jobs whose scan grid exceeds what the
+ * device holds at once: -1 (default) chooses per job (one wave per tile
+ * above 128 survivor regions per CU, else one wave per region), 0 forces one
+ * wave per region, 1 one wave per tile.  Both forms give the same tables.
+ * Applies to single-pair, batched and learned-fallback jobs of the context.
+ * No counterpart in the reference. */
+int v3dm_set_insert_form(v3dm_ctx *ctx, int form);
+/* Form the last single-pair or batched int8 job's insertion pass ran (0 or
+ * 1), or -1 when its scan applied the survivors itself (or before any job). */
+int v3dm_get_last_insert_form(const v3dm_ctx *ctx, int *form);
+
 /* Kernel selection (V3DM_KERNEL_*) and the kernel the last match call ran. */
 int v3dm_set_kernel(v3dm_ctx *ctx, int kernel);
 int v3dm_get_last_kernel(const v3dm_ctx *ctx, int *kernel);

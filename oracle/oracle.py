@@ -27,6 +27,8 @@ _SIG = {
     "orc_scores_u8_as_f32": (None, [ctypes.c_int, _u8, ctypes.c_int, _u8, _f32]),
     "orc_scores_f32": (None, [ctypes.c_int, _f32, ctypes.c_int, _f32, _f32]),
     "orc_scores_f64": (None, [ctypes.c_int, _f32, ctypes.c_int, _f32, _f64]),
+    "orc_scores_f32_simple": (None, [ctypes.c_int, _f32, ctypes.c_int, _f32, _f32]),
+    "orc_scores_f64_simple": (None, [ctypes.c_int, _f32, ctypes.c_int, _f32, _f64]),
     "orc_epipolar_masked": (ctypes.c_int, [_f32, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                            ctypes.c_float]),
     "orc_homography_masked": (ctypes.c_int, [_f32, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
@@ -43,6 +45,8 @@ _SIG = {
                                     ctypes.c_int, _int, _f32, _f32, _f32, _f32, _i32, _i32, _i32]),
     "orc_find_f64": (ctypes.c_int, [ctypes.c_int, _f32, ctypes.c_int, _f32, ctypes.c_double, ctypes.c_double,
                                     _i32, _i32, _i32, _f64, _f64]),
+    "orc_select_f64": (None, [ctypes.c_int, ctypes.c_int, _f64, ctypes.c_double, ctypes.c_double, _i32, _i32, _i32,
+                              _f64, _f64]),
     "orc_find_u8_blocked": (ctypes.c_int, [ctypes.c_int, _u8, ctypes.c_int, _u8, ctypes.c_float, ctypes.c_float,
                                            ctypes.c_int, _i32, _i32, _i32]),
     "orc_max_threads": (ctypes.c_int, []),
@@ -171,6 +175,38 @@ def scores_f32(L: np.ndarray, R: np.ndarray) -> np.ndarray:
     S = np.empty((R.shape[0], L.shape[0]), np.float32)
     lib().orc_scores_f32(L.shape[0], _p(L, _f32), R.shape[0], _p(R, _f32), _p(S, _f32))
     return S
+
+
+def scores_f32_simple(L: np.ndarray, R: np.ndarray) -> np.ndarray:
+    """scores_f32 one dot product at a time (the plain statement of the order)."""
+    L = np.ascontiguousarray(L, np.float32)
+    R = np.ascontiguousarray(R, np.float32)
+    S = np.empty((R.shape[0], L.shape[0]), np.float32)
+    lib().orc_scores_f32_simple(L.shape[0], _p(L, _f32), R.shape[0], _p(R, _f32), _p(S, _f32))
+    return S
+
+
+def scores_f64(L: np.ndarray, R: np.ndarray, simple: bool = False) -> np.ndarray:
+    """FP64 scores S[r, l] of float descriptors (blocked, or one at a time)."""
+    L = np.ascontiguousarray(L, np.float32)
+    R = np.ascontiguousarray(R, np.float32)
+    S = np.empty((R.shape[0], L.shape[0]), np.float64)
+    f = lib().orc_scores_f64_simple if simple else lib().orc_scores_f64
+    f(L.shape[0], _p(L, _f32), R.shape[0], _p(R, _f32), _p(S, _f64))
+    return S
+
+
+def select_f64(S: np.ndarray, min_score: float, min_ratio: float):
+    """The FP64 rule (find_f64's passes) on an FP64 score matrix S[r, l];
+    returns (colMax, rowMax, matchOfLeft)."""
+    S = np.ascontiguousarray(S, np.float64)
+    nR, nL = S.shape
+    col = np.empty(nL, np.int32)
+    row = np.empty(nR, np.int32)
+    tab = np.empty(nL, np.int32)
+    lib().orc_select_f64(nL, nR, _p(S, _f64), min_score, min_ratio, _p(col, _i32), _p(row, _i32), _p(tab, _i32),
+                         None, None)
+    return col, row, tab
 
 
 def select_f32(S: np.ndarray, min_score: float, min_ratio: float):

@@ -89,7 +89,10 @@ void orc_scores_u8_as_f32(int nL, const uint8_t *L, int nR, const uint8_t *R, fl
          S[(size_t)r * nL + l] = (float)dot_u8(L + (size_t)l * ORC_D, R + (size_t)r * ORC_D) * ORC_SCALE;
 }
 
-void orc_scores_f32(int nL, const float *L, int nR, const float *R, float *S)
+/* One dot product at a time (dot_f32_seq / dot_f64): the plain statement of
+ * the score order, which the blocked functions below must equal bit for bit
+ * (tests/test_oracle.py compares them). */
+void orc_scores_f32_simple(int nL, const float *L, int nR, const float *R, float *S)
 {
 #pragma omp parallel for schedule(static)
    for (int r = 0; r < nR; ++r)
@@ -97,12 +100,85 @@ void orc_scores_f32(int nL, const float *L, int nR, const float *R, float *S)
          S[(size_t)r * nL + l] = dot_f32_seq(L + (size_t)l * ORC_D, R + (size_t)r * ORC_D);
 }
 
-void orc_scores_f64(int nL, const float *L, int nR, const float *R, double *S)
+void orc_scores_f64_simple(int nL, const float *L, int nR, const float *R, double *S)
 {
 #pragma omp parallel for schedule(static)
    for (int r = 0; r < nR; ++r)
       for (int l = 0; l < nL; ++l)
          S[(size_t)r * nL + l] = dot_f64(L + (size_t)l * ORC_D, R + (size_t)r * ORC_D);
+}
+
+/* The same scores in blocks of 4 lefts x 8 rights: every one of the 32 dot
+ * products keeps its own sequential order (acc = fmaf(a[k], b[k], acc),
+ * k = 0..127, from +0), the block only runs the independent chains side by
+ * side so they vectorise across the 8 rights. */
+#define ORC_BR 8
+#define ORC_BL 4
+void orc_scores_f32(int nL, const float *L, int nR, const float *R, float *S)
+{
+#pragma omp parallel
+   {
+      float Rt[ORC_D][ORC_BR];
+#pragma omp for schedule(dynamic, 4)
+      for (int r0 = 0; r0 < nR; r0 += ORC_BR)
+      {
+         const int nr = nR - r0 < ORC_BR ? nR - r0 : ORC_BR;
+         for (int k = 0; k < ORC_D; ++k)
+            for (int j = 0; j < ORC_BR; ++j) Rt[k][j] = j < nr ? R[(size_t)(r0 + j) * ORC_D + k] : 0.0f;
+         int l = 0;
+         for (; l + ORC_BL <= nL; l += ORC_BL)
+         {
+            float acc[ORC_BL][ORC_BR];
+            for (int i = 0; i < ORC_BL; ++i)
+               for (int j = 0; j < ORC_BR; ++j) acc[i][j] = 0.0f;
+            for (int k = 0; k < ORC_D; ++k)
+               for (int i = 0; i < ORC_BL; ++i)
+               {
+                  const float a = L[(size_t)(l + i) * ORC_D + k];
+                  for (int j = 0; j < ORC_BR; ++j) acc[i][j] = fmaf(a, Rt[k][j], acc[i][j]);
+               }
+            for (int j = 0; j < nr; ++j)
+               for (int i = 0; i < ORC_BL; ++i) S[(size_t)(r0 + j) * nL + l + i] = acc[i][j];
+         }
+         for (; l < nL; ++l)
+            for (int j = 0; j < nr; ++j)
+               S[(size_t)(r0 + j) * nL + l] = dot_f32_seq(L + (size_t)l * ORC_D, R + (size_t)(r0 + j) * ORC_D);
+      }
+   }
+}
+
+/* FP64 scores, blocked the same way (acc += (double)a[k] * (double)b[k]). */
+void orc_scores_f64(int nL, const float *L, int nR, const float *R, double *S)
+{
+#pragma omp parallel
+   {
+      double Rt[ORC_D][ORC_BR];
+#pragma omp for schedule(dynamic, 4)
+      for (int r0 = 0; r0 < nR; r0 += ORC_BR)
+      {
+         const int nr = nR - r0 < ORC_BR ? nR - r0 : ORC_BR;
+         for (int k = 0; k < ORC_D; ++k)
+            for (int j = 0; j < ORC_BR; ++j) Rt[k][j] = j < nr ? (double)R[(size_t)(r0 + j) * ORC_D + k] : 0.0;
+         int l = 0;
+         for (; l + ORC_BL <= nL; l += ORC_BL)
+         {
+            double acc[ORC_BL][ORC_BR];
+            for (int i = 0; i < ORC_BL; ++i)
+               for (int j = 0; j < ORC_BR; ++j) acc[i][j] = 0.0;
+            for (int k = 0; k < ORC_D; ++k)
+               for (int i = 0; i < ORC_BL; ++i)
+               {
+                  const double a = (double)L[(size_t)(l + i) * ORC_D + k];
+                  for (int j = 0; j < ORC_BR; ++j) acc[i][j] += a * Rt[k][j];
+               }
+            for (int j = 0; j < nr; ++j)
+               for (int i = 0; i < ORC_BL; ++i) S[(size_t)(r0 + j) * nL + l + i] = acc[i][j];
+         }
+         for (; l < nL; ++l)
+            for (int j = 0; j < nr; ++j)
+               S[(size_t)(r0 + j) * nL + l] = dot_f64(L + (size_t)l * ORC_D, R + (size_t)(r0 + j) * ORC_D);
+      }
+   }
 }
 
 /* ------------------------------------------------------------------------ */
@@ -184,21 +260,37 @@ static int orc_accept(float maxVal, float nextVal, float threshold, float minRat
 void orc_column_pass(int nR, int nL, float threshold, float minRatio, const float *S, int32_t *argMaxOut,
                      float *bestOut, float *nextOut)
 {
-#pragma omp parallel for schedule(static)
-   for (int col = 0; col < nL; ++col)
+   /* 64 adjacent columns at a time (each still scans its rows in increasing
+    * order), so every row contributes one contiguous 256-byte read */
+#pragma omp parallel for schedule(dynamic, 1)
+   for (int col0 = 0; col0 < nL; col0 += 64)
    {
-      float maxVal = ORC_INIT, nextVal = ORC_INIT;
-      int argMax = -1;
+      const int nc = nL - col0 < 64 ? nL - col0 : 64;
+      float maxVal[64], nextVal[64];
+      int argMax[64];
+      for (int c = 0; c < 64; ++c)
+      {
+         maxVal[c] = ORC_INIT;
+         nextVal[c] = ORC_INIT;
+         argMax[c] = -1;
+      }
       for (int row = 0; row < nR; ++row)
       {
-         float val = S[(size_t)row * nL + col];
-         argMax = (val > maxVal) ? row : argMax;
-         nextVal = (val > maxVal) ? maxVal : fmaxf(val, nextVal);
-         maxVal = fmaxf(maxVal, val);
+         const float *Srow = S + (size_t)row * nL + col0;
+         for (int c = 0; c < nc; ++c)
+         {
+            float val = Srow[c];
+            argMax[c] = (val > maxVal[c]) ? row : argMax[c];
+            nextVal[c] = (val > maxVal[c]) ? maxVal[c] : fmaxf(val, nextVal[c]);
+            maxVal[c] = fmaxf(maxVal[c], val);
+         }
       }
-      argMaxOut[col] = orc_accept(maxVal, nextVal, threshold, minRatio) ? argMax : -1;
-      if (bestOut) bestOut[col] = maxVal;
-      if (nextOut) nextOut[col] = nextVal;
+      for (int c = 0; c < nc; ++c)
+      {
+         argMaxOut[col0 + c] = orc_accept(maxVal[c], nextVal[c], threshold, minRatio) ? argMax[c] : -1;
+         if (bestOut) bestOut[col0 + c] = maxVal[c];
+         if (nextOut) nextOut[col0 + c] = nextVal[c];
+      }
    }
 }
 
@@ -308,27 +400,39 @@ static int orc_accept_f64(double maxVal, double nextVal, double threshold, doubl
    return maxVal > threshold && bestDist / nextDist < minRatio;
 }
 
-int orc_find_f64(int nL, const float *L, int nR, const float *R, double minScore, double minRatio,
-                 int32_t *colMax, int32_t *rowMax, int32_t *matchOfLeft, double *bestL, double *nextL)
+/* Column + row passes and the mutual check on an FP64 score matrix S[r*nL + l]. */
+void orc_select_f64(int nL, int nR, const double *S, double minScore, double minRatio, int32_t *colMax,
+                    int32_t *rowMax, int32_t *matchOfLeft, double *bestL, double *nextL)
 {
-   double *S = (double *)malloc(sizeof(double) * (size_t)nL * (size_t)nR + 8);
-   if (!S) return -1;
-   orc_scores_f64(nL, L, nR, R, S);
-#pragma omp parallel for schedule(static)
-   for (int col = 0; col < nL; ++col)
+#pragma omp parallel for schedule(dynamic, 1)
+   for (int col0 = 0; col0 < nL; col0 += 64)
    {
-      double maxVal = -1e30, nextVal = -1e30;
-      int argMax = -1;
+      const int nc = nL - col0 < 64 ? nL - col0 : 64;
+      double maxVal[64], nextVal[64];
+      int argMax[64];
+      for (int c = 0; c < 64; ++c)
+      {
+         maxVal[c] = -1e30;
+         nextVal[c] = -1e30;
+         argMax[c] = -1;
+      }
       for (int row = 0; row < nR; ++row)
       {
-         double val = S[(size_t)row * nL + col];
-         argMax = (val > maxVal) ? row : argMax;
-         nextVal = (val > maxVal) ? maxVal : fmax(val, nextVal);
-         maxVal = fmax(maxVal, val);
+         const double *Srow = S + (size_t)row * nL + col0;
+         for (int c = 0; c < nc; ++c)
+         {
+            double val = Srow[c];
+            argMax[c] = (val > maxVal[c]) ? row : argMax[c];
+            nextVal[c] = (val > maxVal[c]) ? maxVal[c] : fmax(val, nextVal[c]);
+            maxVal[c] = fmax(maxVal[c], val);
+         }
       }
-      colMax[col] = orc_accept_f64(maxVal, nextVal, minScore, minRatio) ? argMax : -1;
-      if (bestL) bestL[col] = maxVal;
-      if (nextL) nextL[col] = nextVal;
+      for (int c = 0; c < nc; ++c)
+      {
+         colMax[col0 + c] = orc_accept_f64(maxVal[c], nextVal[c], minScore, minRatio) ? argMax[c] : -1;
+         if (bestL) bestL[col0 + c] = maxVal[c];
+         if (nextL) nextL[col0 + c] = nextVal[c];
+      }
    }
 #pragma omp parallel for schedule(static)
    for (int row = 0; row < nR; ++row)
@@ -345,6 +449,15 @@ int orc_find_f64(int nL, const float *L, int nR, const float *R, double minScore
       rowMax[row] = orc_accept_f64(maxVal, nextVal, minScore, minRatio) ? argMax : -1;
    }
    orc_mutual_table(nL, colMax, rowMax, matchOfLeft);
+}
+
+int orc_find_f64(int nL, const float *L, int nR, const float *R, double minScore, double minRatio,
+                 int32_t *colMax, int32_t *rowMax, int32_t *matchOfLeft, double *bestL, double *nextL)
+{
+   double *S = (double *)malloc(sizeof(double) * (size_t)nL * (size_t)nR + 8);
+   if (!S) return -1;
+   orc_scores_f64(nL, L, nR, R, S);
+   orc_select_f64(nL, nR, S, minScore, minRatio, colMax, rowMax, matchOfLeft, bestL, nextL);
    free(S);
    return 0;
 }

@@ -157,6 +157,23 @@ def test_f32_scores_are_sequential_fma():
             assert S[r, l] == F32(acc)
 
 
+@pytest.mark.parametrize("shape", [(37, 29), (64, 8), (5, 3), (130, 61)])
+def test_blocked_float_scores_equal_one_at_a_time(shape):
+    """The blocked FP32 / FP64 score functions (4 lefts x 8 rights side by
+    side) must give every dot product bit for bit as the one-at-a-time loop,
+    also for signs, zeros, subnormals and large magnitudes."""
+    nL, nR = shape
+    rng = np.random.default_rng(nL * 1000 + nR)
+    L = rng.standard_normal((nL, 128)).astype(F32)
+    R = rng.standard_normal((nR, 128)).astype(F32)
+    L[::3, ::7] = F32(0.0)
+    L[1::5, 3::11] = F32(-0.0)
+    R[::4, 5::9] = F32(1e-40)  # subnormal
+    R[2::7, ::13] = F32(3e18)
+    np.testing.assert_array_equal(O.scores_f32(L, R).view(np.uint32), O.scores_f32_simple(L, R).view(np.uint32))
+    np.testing.assert_array_equal(O.scores_f64(L, R).view(np.uint64), O.scores_f64(L, R, simple=True).view(np.uint64))
+
+
 # ---- hand-computed special cases ------------------------------------------------
 
 def _col(S, ms=0.8, mr=0.8):

@@ -630,16 +630,37 @@ hipError_t launch_i8_scan(int variant, int grid, const ScanArgs &a, hipStream_t 
    return hipGetLastError();
 }
 
-hipError_t launch_i8_insert(const ScanArgs &a, hipStream_t st)
+// Insertion form for `regions` survivor regions on `device` when the caller
+// leaves the choice open (form < 0): one wave per tile above 128 regions per
+// CU (on 256 CUs: 32768 -- the 256-pair batch's 147 k regions ran 127 -> 116 us
+// per tile, 8192^2 with 256 x 256 tiles, 4096 regions, 4.0 us per region
+// against 7.3 us per tile), else one wave per region.
+int insert_form_for(int64_t regions, int device, int form)
+{
+   if (form == 0 || form == 1) return form;
+   static std::atomic<int> cusKnown[kMaxDevicesCached];
+   int cus = device >= 0 && device < kMaxDevicesCached ? cusKnown[device].load(std::memory_order_relaxed) : 0;
+   if (cus <= 0)
+   {
+      if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0)
+         cus = 256;
+      if (device >= 0 && device < kMaxDevicesCached) cusKnown[device].store(cus, std::memory_order_relaxed);
+   }
+   return regions > (int64_t)128 * cus ? 1 : 0;
+}
+
+hipError_t launch_i8_insert(const ScanArgs &a, int form, hipStream_t st)
 {
    if (a.nTiles <= 0) return hipSuccess;
    if (a.survCap < 64) return hipErrorInvalidValue; // each wave reads its regions' first 64 entries
    if (a.waves < 1 || a.waves > kMaxScanWaves) return hipErrorInvalidValue;
    const int64_t regions = (int64_t)a.nTiles * a.waves;
-   if (regions > 32768) // 256-pair batches (36 k tiles): 127 -> 116 us per tile
+   if (form == 1)
       hipLaunchKernelGGL(k_i8_insert<true>, dim3((a.nTiles + 3) / 4), dim3(256), 0, st, a);
-   else // 8192^2 with 256 x 256 tiles (1024 tiles): 4.0 us per region, 7.3 us per tile
+   else if (form == 0)
       hipLaunchKernelGGL(k_i8_insert<false>, dim3((unsigned)((regions + 3) / 4)), dim3(256), 0, st, a);
+   else
+      return hipErrorInvalidValue;
    return hipGetLastError();
 }
 

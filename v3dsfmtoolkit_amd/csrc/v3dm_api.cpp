@@ -322,7 +322,12 @@ int enqueueI8Job(v3dm_ctx *c, int v, float minScore, float minRatio, bool marks)
       HIPCHK(c, hipEventRecord(e1, c->stream));
    }
    HIPCHK(c, launch_i8_scan(v, a.nTiles, a, c->stream, e0, e1));
-   if (!a.inlineInsert) HIPCHK(c, launch_i8_insert(a, c->stream));
+   c->lastInsertForm = -1;
+   if (!a.inlineInsert)
+   {
+      c->lastInsertForm = insert_form_for((int64_t)a.nTiles * a.waves, c->device, c->insertForm);
+      HIPCHK(c, launch_i8_insert(a, c->lastInsertForm, c->stream));
+   }
    FinalizeArgs f{};
    f.pairs = c->dPair;
    f.nPairs = 1;
@@ -847,6 +852,21 @@ int v3dm_get_last_scan_inline(const v3dm_ctx *c, int *inlineInsert)
 {
    if (!c || !inlineInsert) return V3DM_ERR_ARG;
    *inlineInsert = c->lastInline;
+   return V3DM_OK;
+}
+
+int v3dm_set_insert_form(v3dm_ctx *c, int form)
+{
+   if (!c) return V3DM_ERR_ARG;
+   if (form < -1 || form > 1) return fail(c, V3DM_ERR_ARG, "set_insert_form: form %d out of range", form);
+   c->insertForm = form;
+   return V3DM_OK;
+}
+
+int v3dm_get_last_insert_form(const v3dm_ctx *c, int *form)
+{
+   if (!c || !form) return V3DM_ERR_ARG;
+   *form = c->lastInsertForm;
    return V3DM_OK;
 }
 

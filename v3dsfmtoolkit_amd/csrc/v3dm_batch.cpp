@@ -309,7 +309,12 @@ int enqueueKernels(v3dm_ctx *c, BatchState *b, float minScore, float minRatio, b
    int rc;
    if (marks && (rc = profMark(c, 0))) return rc;
    HIPCHK(c, launch_i8_scan(b->variant, grid, a, c->stream));
-   if (!a.inlineInsert) HIPCHK(c, launch_i8_insert(a, c->stream));
+   c->lastInsertForm = -1;
+   if (!a.inlineInsert)
+   {
+      c->lastInsertForm = insert_form_for((int64_t)a.nTiles * a.waves, c->device, c->insertForm);
+      HIPCHK(c, launch_i8_insert(a, c->lastInsertForm, c->stream));
+   }
    if (marks && (rc = profMark(c, 1))) return rc;
    HIPCHK(c, launch_i8_finalize(f, c->stream));
    HIPCHK(c, launch_advance_epoch(b->dCounters, b->dCounters + 2, c->stream));

@@ -21,7 +21,8 @@ int scan_variant_waves(int v);
 int scan_variant_tile_m(int v);
 int scan_variant_tile_n(int v);
 int scan_variant_resident(int v, int device);
-hipError_t launch_i8_insert(const ScanArgs &a, hipStream_t st);
+int insert_form_for(int64_t regions, int device, int form);
+hipError_t launch_i8_insert(const ScanArgs &a, int form, hipStream_t st);
 hipError_t launch_i8_finalize(const FinalizeArgs &a, hipStream_t st);
 hipError_t launch_bf16_scan(int variant, int grid, const ScanArgsF &a, hipStream_t st, hipEvent_t start = nullptr,
                             hipEvent_t stop = nullptr);
@@ -180,6 +181,8 @@ struct v3dm_ctx
    int scanVariant = -1;                                 // single-pair int8 scan variant, -1: measured
    int lastVariant = -1;                                 // variant of the last single-pair int8 job
    int lastInline = -1;                                  // its scan applied the survivors itself (1) or not (0)
+   int insertForm = -1;      // insertion pass: -1 chosen per job, 0 one wave per region, 1 one wave per tile
+   int lastInsertForm = -1;  // form the last int8 job's insertion pass ran (-1: none, survivors applied in the scan)
    std::vector<std::pair<uint64_t, int>> tunedI8;       // shape bucket -> fastest variant
 
    // float path

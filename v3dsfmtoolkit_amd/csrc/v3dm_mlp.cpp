@@ -599,7 +599,8 @@ int v3dm_learned_find(v3dm_ctx *c, v3dm_mlp *m, int topK, float margin, float mi
          a.mapR = side == 0 ? nullptr : s->flagR;
          a.inlineInsert = a.nTiles <= scan_variant_resident(0, c->device) ? 1 : 0;
          HIPCHK(c, launch_i8_scan(0, a.nTiles, a, st));
-         if (!a.inlineInsert) HIPCHK(c, launch_i8_insert(a, st));
+         if (!a.inlineInsert)
+            HIPCHK(c, launch_i8_insert(a, insert_form_for((int64_t)a.nTiles * a.waves, c->device, c->insertForm), st));
       }
    }
    // 6. the exact matcher's ratio test and mutual check on the scored pairs

@@ -91,6 +91,16 @@ class SIFTMatching:
         check(self._lib.v3dm_get_last_scan_inline(self._ctx, ctypes.byref(k)), self._ctx)
         return k.value
 
+    def setInsertForm(self, form: int) -> None:
+        """Insertion pass of large int8 jobs: -1 automatic, 0 one wave per region, 1 one wave per tile."""
+        check(self._lib.v3dm_set_insert_form(self._ctx, int(form)), self._ctx)
+
+    def lastInsertForm(self) -> int:
+        """Form of the last int8 job's insertion pass (0 / 1), -1 when the scan applied its survivors."""
+        k = ctypes.c_int(-1)
+        check(self._lib.v3dm_get_last_insert_form(self._ctx, ctypes.byref(k)), self._ctx)
+        return k.value
+
     def lastKernel(self) -> int:
         k = ctypes.c_int(0)
         check(self._lib.v3dm_get_last_kernel(self._ctx, ctypes.byref(k)), self._ctx)
