@@ -159,6 +159,27 @@ int v3dm_fetch_matches(v3dm_ctx *ctx, int32_t *pairs, int capacity, int *count);
 
 int v3dm_synchronize(v3dm_ctx *ctx);
 
+/* --- the same job with no device allocation ------------------------------ */
+
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  findPutativeMatches of one uint8 pair whose
+ * descriptors are already in device memory: dLeft (nLeft x 128) and dRight
+ * (nRight x 128), row-major on the current HIP device; counts are rounded
+ * down to multiples of 32 as setLeftFeatures does.  Writes the per-left
+ * table (right index or -1, the reference's mutual list,
+ * v3d_cudamatching.cpp:121-126) to dMatchOfLeft[0 .. rounded nLeft) on the
+ * device.  Enqueued on `stream` (a hipStream_t, NULL = the null stream) and
+ * returns at once; the caller synchronises.  The working memory is a
+ * 3.4 MiB array of the library in each GPU's memory (held until the process
+ * ends; the first call on a device looks up its address and clears it with
+ * a kernel), so at most 8192 features per side (V3DM_ERR_CAPACITY beyond);
+ * nothing is allocated by the call.  Every call on one GPU uses that GPU's
+ * array: calls that overlap on one GPU (other streams, other threads) race.
+ * Errors are reported by v3dm_last_error(NULL). */
+int v3dm_find_u8_noalloc(const uint8_t *dLeft, int nLeft, const uint8_t *dRight, int nRight, float minScore,
+                         float minRatio, int32_t *dMatchOfLeft, void *stream);
+
 /* --- kernel timing (HIP events recorded on the context stream) ------------ */
 
 /* When enabled, every match job records HIP events around its kernels

@@ -79,6 +79,8 @@ _SIGNATURES = {
     "v3dm_batch_info": (ctypes.c_int, [_ctx_p, _c_int_p, _i64_p, _c_int_p, _i64_p]),
     "v3dm_scan_variant_count": (ctypes.c_int, []),
     "v3dm_set_insert_form": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "v3dm_find_u8_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                            ctypes.c_float, ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p]),
     "v3dm_get_last_insert_form": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
     "v3dm_scan_variant_info": (ctypes.c_int, [ctypes.c_int, _c_int_p, _c_int_p, _c_int_p]),
     "v3dm_synth_pair": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,

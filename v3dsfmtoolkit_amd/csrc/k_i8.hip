@@ -432,13 +432,13 @@ __global__ __launch_bounds__(256) void k_i8_insert(ScanArgs a)
          w0[w] = a.surv[(r0 + w) * a.survCap + lane];
       }
    }
-   const int p = pair_of_tile(a.tilePair, a.pairs, a.nPairs, tile);
+   const int p = a.nPairs > 1 ? pair_of_tile(a.tilePair, a.pairs, a.nPairs, tile) : 0;
    const uint32_t epoch = *a.epoch;
    int any = 0;
 #pragma unroll
    for (int w = 0; w < kSpan; ++w) any |= cnt[w];
    if (any == 0) return;
-   const PairDesc pd = a.pairs[p];
+   const PairDesc pd = a.nPairs > 1 ? a.pairs[p] : a.pair0; // a single pair comes by value
    const int t = tile - pd.tileBase;
    const int tm = t / pd.tilesN, tn = t - tm * pd.tilesN;
    const int c0 = tn * a.tileN;
