@@ -83,19 +83,15 @@ __device__ inline int max16(const v16i &x)
 // steps cover all 128.  Accumulator register v of lane l is
 // D[8*(v/4) + 4*(l/32) + v%4][l%32].
 //
-// Tiling.  A workgroup of 4 waves owns 256 left rows (64 per wave, held in
-// VGPRs for the whole tile) and sweeps 512 right rows in steps of 32; two
-// workgroups share a CU, so one computes while the other waits at a
-// barrier.  The right rows are staged through LDS once per workgroup and
-// read by all 4 waves.  Staging works in groups of 4 steps (16 KiB): while the waves
-// compute group g from one third of a 3-group LDS ring, the group g+1 data
-// (loaded during group g-1) is written to the next third and group g+2 is
-// fetched into registers.  One barrier per group; the third written at group
-// g was last read at group g-2, which every wave finished before the
-// barrier of group g-1.  Within a group the waves run free, and each wave
-// reads the fragments of step s+1 and issues its MFMAs before it tests the
-// scores of step s (two accumulator sets), so the test overlaps the matrix
-// pipeline.
+// Tiling.  A workgroup of 4 (or 8) waves owns 256 (512) left rows, 64 per
+// wave held in VGPRs as MFMA A operands for the whole tile, and sweeps
+// TILE_N right rows in steps of 32; two workgroups share a CU (256 VGPRs
+// each), so one computes while the other waits at a barrier.  The right rows
+// are staged through LDS once per workgroup and read by all its waves, in
+// groups of 4 steps (16 KiB) with one barrier per group.  Within a group the
+// waves run free, and each wave reads the fragments of step s+1 and issues
+// its MFMAs before it tests the scores of step s (two accumulator sets), so
+// the test overlaps the matrix pipeline.
 // LDS unit layout: 16-byte unit p = row * 8 + (chunk ^ swz(row)),
 // swz(r) = (r ^ r >> 3) & 7, so the 32 rows of one fragment read (same
 // chunk) fall in distinct bank groups for 8- and 16-lane LDS service.
@@ -104,23 +100,109 @@ constexpr int kStageBytes = kStepCols * kDim;      // 4 KiB
 constexpr int kGroup = 4;                          // steps per barrier
 constexpr int kGroupBytes = kGroup * kStageBytes;  // 16 KiB
 
-// One workgroup per tile, or (a.persistent) resident workgroups that take
-// tiles blockIdx.x, then gridDim.x + atomicAdd(tileCounter) until the tiles
-// run out; the next index is requested when a tile starts, so its latency
-// hides behind the tile.
+// Right rows reach LDS by direct-to-LDS loads (global_load_lds_dwordx4: each
+// lane's 16 bytes land at LDS base + 16 * lane), so no register ring and no
+// LDS stores: the lane that loads LDS unit p fetches the global 16 bytes the
+// swizzled layout puts there.  Four 16 KiB buffers per workgroup (distinct
+// arrays, each named at compile time: the group loop runs in fours); group
+// g+3 is loaded right after the barrier of group g into the buffer group g-1
+// used.  One workgroup per tile, or (a.persistent) resident workgroups that
+// take tiles blockIdx.x, then gridDim.x + atomicAdd(tileCounter) until the
+// tiles run out; the next index is requested when a tile starts.
+// At that barrier a wave waits until only its loads of group g+2 may be in
+// flight (the group g+1 data must be there: the last step of group g issues
+// the MFMAs of group g+1's first step), then the barrier makes every wave's
+// loads visible.
+constexpr int kBufs = 4;
+
+template <int N> __device__ __forceinline__ void vm_wait_barrier()
+{
+   asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)\n\ts_barrier" ::"n"(N) : "memory");
+}
+
+// Direct-to-LDS loads, written out so the compiler's wait placement does not
+// see them: it cannot tell, after the survivor branch rejoins, which buffer a
+// pending load fills, and would wait for all of them before every LDS read.
+// The waits these loads need are the explicit ones at the group barriers;
+// the compiler's own vmcnt waits (for its own loads) only get stricter, as
+// vector memory operations complete in issue order.  M0 holds the
+// wave-uniform LDS address (one wait state after it is written) and gets its
+// previous value back afterwards; lane l writes bytes [16 l, 16 l + 16)
+// (dwordx4) or [4 l, 4 l + 4) (dword) there.
+__device__ __forceinline__ void lds_dma16(const void *base, int voff, uint32_t ldsAddr)
+{
+   uint32_t saved;
+   asm volatile("s_mov_b32 %0, m0\n\ts_mov_b32 m0, %3\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, %2\n\t"
+                "s_nop 0\n\ts_mov_b32 m0, %0"
+                : "=&s"(saved)
+                : "v"(voff), "s"(base), "s"(ldsAddr)
+                : "memory");
+}
+__device__ __forceinline__ void lds_dma4(const void *base, int voff, uint32_t ldsAddr)
+{
+   uint32_t saved;
+   asm volatile("s_mov_b32 %0, m0\n\ts_mov_b32 m0, %3\n\ts_nop 0\n\tglobal_load_lds_dword %1, %2\n\t"
+                "s_nop 0\n\ts_mov_b32 m0, %0"
+                : "=&s"(saved)
+                : "v"(voff), "s"(base), "s"(ldsAddr)
+                : "memory");
+}
+__device__ __forceinline__ uint32_t lds_addr(const void *p)
+{
+   return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void *)p;
+}
+// a wave-uniform pointer in scalar registers (the asm operands above need SGPRs)
+__device__ __forceinline__ const void *uniform_ptr(const void *p)
+{
+   const uint64_t u = (uint64_t)(uintptr_t)p;
+   return (const void *)(uintptr_t)(((uint64_t)(uint32_t)sgpr((int)(u >> 32)) << 32) | (uint32_t)sgpr((int)u));
+}
+
 template <int WAVES, int TILE_N, bool INLINE_INSERT>
 __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(2))) void k_i8_scan(ScanArgs a)
 {
    constexpr int kScanWaves = WAVES;
    constexpr int kTileM = WAVES * kWaveRows;
    constexpr int kTileN = TILE_N;
-   __shared__ __attribute__((aligned(16))) int8_t sB[kThirds][kGroupBytes];
-   __shared__ __attribute__((aligned(16))) int32_t sC[kThirds][kGroup * kStepCols];
+   constexpr int kDmaB = kGroupBytes / 1024 / WAVES;             // 16-byte loads of a group's rows per wave
+   constexpr int kDmaC = kGroup * kStepCols * 4 / 256;            // 4-byte loads of a group's column terms
+   constexpr int kDma = kDmaB + kDmaC;                            // loads per wave per group
+   static_assert(kGroupBytes % (1024 * WAVES) == 0, "a group must split into whole 1 KiB loads per wave");
+   __shared__ __attribute__((aligned(16))) int8_t sB0[kGroupBytes];
+   __shared__ __attribute__((aligned(16))) int8_t sB1[kGroupBytes];
+   __shared__ __attribute__((aligned(16))) int8_t sB2[kGroupBytes];
+   __shared__ __attribute__((aligned(16))) int8_t sB3[kGroupBytes];
+   __shared__ __attribute__((aligned(16))) int32_t sC0[kGroup * kStepCols];
+   __shared__ __attribute__((aligned(16))) int32_t sC1[kGroup * kStepCols];
+   __shared__ __attribute__((aligned(16))) int32_t sC2[kGroup * kStepCols];
+   __shared__ __attribute__((aligned(16))) int32_t sC3[kGroup * kStepCols];
    __shared__ int sNextTile;
+   auto bufB = [&](auto bc) -> int8_t * {
+      constexpr int b = decltype(bc)::value;
+      if constexpr (b == 0) return sB0;
+      else if constexpr (b == 1) return sB1;
+      else if constexpr (b == 2) return sB2;
+      else return sB3;
+   };
+   auto bufC = [&](auto bc) -> int32_t * {
+      constexpr int b = decltype(bc)::value;
+      if constexpr (b == 0) return sC0;
+      else if constexpr (b == 1) return sC1;
+      else if constexpr (b == 2) return sC2;
+      else return sC3;
+   };
 
    const int tid = threadIdx.x;
    const int lane = tid & 63;
    const int wave = tid >> 6;
+   // this lane's global offset (within a group) of each of its row loads
+   int voffd[kDmaB];
+#pragma unroll
+   for (int j = 0; j < kDmaB; ++j)
+   {
+      const int p = (wave * kDmaB + j) * 64 + lane, st = p >> 8, pu = p & 255, r = pu >> 3, x = pu & 7;
+      voffd[j] = st * kStageBytes + r * kDim + ((x ^ swz(r)) << 4);
+   }
    int tile = blockIdx.x;
    while (tile < a.nTiles)
    {
@@ -151,51 +233,29 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(2)))
    const int ngroups = nsteps / kGroup;
    const int8_t *Rt = a.R + (size_t)(pd.offR + c0) * kDim;
    const int32_t *St = a.Rsum + pd.offR + c0;
-   const __amdgpu_buffer_rsrc_t rsB = __builtin_amdgcn_make_buffer_rsrc((void *)Rt, (short)0, nsteps * kStageBytes,
-                                                                         kRsrcWord3);
-   const __amdgpu_buffer_rsrc_t rsS = __builtin_amdgcn_make_buffer_rsrc((void *)St, (short)0,
-                                                                         nsteps * kStepCols * 4, kRsrcWord3);
    uint64_t *region = a.surv + ((size_t)tile * kScanWaves + wave) * a.survCap;
    int nsurv = 0; // survivors this wave found in this tile (wave-uniform)
 
-   // This thread's 16-byte units of a group (1024 units): global offset, LDS offset.
-   constexpr int kUnits = kGroupBytes / 16 / (kScanWaves * 64);
-   int voff[kUnits], loff[kUnits];
+   // group g's rows and column terms into buffer bc (nothing past the tile)
+   auto dma = [&](auto bc, int g) {
+      if (g >= ngroups) return;
+      const uint32_t db = lds_addr(bufB(bc)), dc = lds_addr(bufC(bc));
+      const void *src = uniform_ptr(Rt + (size_t)g * kGroupBytes);
 #pragma unroll
-   for (int j = 0; j < kUnits; ++j)
-   {
-      const int q = tid + kScanWaves * 64 * j, st = q >> 8, pu = q & 255, r = pu >> 3, x = pu & 7;
-      voff[j] = st * kStageBytes + r * kDim + ((x ^ swz(r)) << 4);
-      loff[j] = st * kStageBytes + pu * 16;
-   }
-   typedef unsigned int v2u __attribute__((ext_vector_type(2)));
-   // Two register rings: group g+1 is stored to LDS while group g+3 is
-   // fetched, so every fetch has two groups of compute to arrive, and the
-   // prologue issues groups 0 and 1 together with the A rows (one round trip).
-   v4i ring[2][kUnits];
-   v2u ringS[2] = {{0u, 0u}, {0u, 0u}};
-   auto fetch = [&](auto slotc, int g) {
-      constexpr int slot = decltype(slotc)::value;
-      const int gc = min(g, ngroups - 1);
+      for (int j = 0; j < kDmaB; ++j) lds_dma16(src, voffd[j], (uint32_t)sgpr((int)(db + (wave * kDmaB + j) * 1024)));
+      const void *srcs = uniform_ptr(St + g * (kGroup * kStepCols));
 #pragma unroll
-      for (int j = 0; j < kUnits; ++j)
-         ring[slot][j] = __builtin_amdgcn_raw_buffer_load_b128(rsB, voff[j], gc * kGroupBytes, 0);
-      // every wave loads the sums: a uniform load count keeps the vmcnt waits exact
-      ringS[slot] = __builtin_amdgcn_raw_buffer_load_b64(rsS, lane * 8, gc * kGroup * kStepCols * 4, 0);
+      for (int j = 0; j < kDmaC; ++j) lds_dma4(srcs, 4 * (64 * j + lane), (uint32_t)sgpr((int)(dc + 256 * j)));
    };
-   auto store = [&](auto slotc, int third) {
-      constexpr int slot = decltype(slotc)::value;
-#pragma unroll
-      for (int j = 0; j < kUnits; ++j) *reinterpret_cast<v4i *>(&sB[third][loff[j]]) = ring[slot][j];
-      // every wave writes the same sums (no branch, so the vmcnt waits stay exact)
-      *reinterpret_cast<v2u *>(&sC[third][2 * lane]) = ringS[slot];
-   };
-   const std::integral_constant<int, 0> R0;
-   const std::integral_constant<int, 1> R1;
+   const std::integral_constant<int, 0> B0;
+   const std::integral_constant<int, 1> B1;
+   const std::integral_constant<int, 2> B2;
+   const std::integral_constant<int, 3> B3;
 
-   fetch(R0, 0);
-   fetch(R1, 1);
-
+   // groups 0 and 1 and the left rows; group 2 follows the first MFMAs (the
+   // compiler's wait for the left rows before them must not include it)
+   dma(B0, 0);
+   dma(B1, 1);
    v4i A[2][4];
    v16i C[2];
    if (active)
@@ -220,13 +280,6 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(2)))
 #pragma unroll
    for (int k = 0; k < 4; ++k) fragoff[k] = (lrow * 8 + ((4 * half + k) ^ swz(lrow))) << 4;
 
-   store(R0, 0);
-   fetch(R0, 2);
-
-   // Software pipeline over the steps of the tile: the MFMAs of the next step
-   // (also across a group boundary) are issued before the survivor test of the
-   // current one, so the test's VALU work overlaps the matrix pipeline instead
-   // of waiting for its result.
    auto mfma_step = [&](const v4i(&Bf)[4], v16i &x0, v16i &x1) {
       x0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[0][0], Bf[0], C[0], 0, 0, 0);
       x1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[1][0], Bf[0], C[1], 0, 0, 0);
@@ -238,31 +291,37 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(2)))
       }
    };
    v16i p0, p1; // scores of the step in flight
-   // `lastc` says at compile time whether g is the tile's last group: only
-   // its last step issues no next-step MFMAs, so the accumulator sets of the
-   // other groups never merge with a not-updated set (which made the
-   // compiler copy one set into the other at every group boundary).
-   auto group = [&](auto slotc, int g, auto lastc) {
+   // `lastc` says at compile time whether g is the tile's last group (only its
+   // last step issues no next-step MFMAs)
+   auto group = [&](auto bc, int g, auto lastc) {
+      constexpr int b = decltype(bc)::value;
       constexpr bool kLast = decltype(lastc)::value;
-      // ring slot `slotc` holds group g+1: stage it, then refill with group g+3
-      store(slotc, (g + 1) % kThirds);
-      fetch(slotc, g + 3);
-      lds_barrier();
-      if (!active) return;
-      const int8_t *sb = sB[g % kThirds];
-      const int32_t *sc = sC[g % kThirds];
+      const std::integral_constant<int, (b + 1) % kBufs> bn;
+      const std::integral_constant<int, (b + 3) % kBufs> bf;
+      // groups <= g+1 (and at g = 0 the left rows) in LDS / registers: only
+      // group g+2's loads may still be in flight (none was issued past the
+      // tile's end, and group 2 is issued after the barrier of group 0), then
+      // the barrier
+      if (g > 0 && g + 2 < ngroups) vm_wait_barrier<kDma>();
+      else vm_wait_barrier<0>();
+      const int8_t *sb = bufB(bc);
+      const int32_t *sc = bufC(bc);
       if (g == 0)
       {
-         v4i B0[4];
+         if (active)
+         {
+            v4i Bz[4];
 #pragma unroll
-         for (int k = 0; k < 4; ++k) B0[k] = *reinterpret_cast<const v4i *>(sb + fragoff[k]);
-         mfma_step(B0, p0, p1);
+            for (int k = 0; k < 4; ++k) Bz[k] = *reinterpret_cast<const v4i *>(sb + fragoff[k]);
+            mfma_step(Bz, p0, p1);
+         }
+         dma(B2, 2);
       }
+      dma(bf, g + 3); // into the buffer of group g-1, which every wave has finished
+      if (!active) return;
 #pragma unroll
       for (int s = 0; s < kGroup; ++s)
       {
-         // Fragments of the next step; after the barrier of group g the stage
-         // of group g+1 is complete too (written before that barrier).
          v4i Bn[4];
          const bool more = s + 1 < kGroup || !kLast; // constant once unrolled
          if (s + 1 < kGroup)
@@ -272,7 +331,7 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(2)))
          }
          else if (more)
          {
-            const int8_t *sn = sB[(g + 1) % kThirds];
+            const int8_t *sn = bufB(bn);
 #pragma unroll
             for (int k = 0; k < 4; ++k) Bn[k] = *reinterpret_cast<const v4i *>(sn + fragoff[k]);
          }
@@ -283,21 +342,16 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(2)))
          const int lim = thr - cs; // score > thr  <=>  acc > thr - 128*sum(u_right)
          if (__builtin_expect(__any(m > lim), 0))
          {
-            // rare: append every surviving score of the wave to its region
-            // (plain stores, nothing waits on them), applied when the tile
-            // is done (see below).
-            // Padding rows and columns never pass (kPadRowSum), so there are
-            // no bounds to check here; blocks of 4 registers are skipped
-            // whole when none of their scores passes.
+            // rare: every surviving score of the wave to its region (padding
+            // rows and columns never pass; 4-register blocks with no passing
+            // score skipped whole)
             const int jrel = (g * kGroup + s) * kStepCols + lrow;
-            // recomputed here on every visit: hoisted out of the loop, the 32
-            // entry constants below would be spilled to scratch
             int half4 = 4 * half;
             asm volatile("" : "+v"(half4));
 #pragma unroll
-            for (int b = 0; b < 2; ++b)
+            for (int bb = 0; bb < 2; ++bb)
             {
-               const v16i &acc = b == 0 ? acc0 : acc1;
+               const v16i &acc = bb == 0 ? acc0 : acc1;
 #pragma unroll
                for (int q = 0; q < 4; ++q)
                {
@@ -310,13 +364,13 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(2)))
                      const bool h = acc[v] > lim;
                      const uint64_t ball = __ballot(h);
                      if (ball == 0) continue;
-                     const int irel = 32 * b + 8 * q + half4 + e;
+                     const int irel = 32 * bb + 8 * q + half4 + e;
                      const int slot = nsurv + __builtin_amdgcn_mbcnt_hi((uint32_t)(ball >> 32),
                                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)ball, 0));
                      if (h && slot < a.survCap)
                         region[slot] = ((uint64_t)(uint32_t)(acc[v] + cs) << 32) | ((uint32_t)irel << 16) |
                                        (uint32_t)jrel;
-                     nsurv += __builtin_popcountll(ball); // counts past survCap: region overflowed
+                     nsurv += __builtin_popcountll(ball);
                   }
                }
             }
@@ -325,26 +379,38 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(2)))
    };
    const std::false_type kMore;
    const std::true_type kEnd;
+   // groups in fours (buffer = group mod 4 at compile time); the tile's last
+   // group is peeled
    int g = 0;
-   for (; g + 2 < ngroups; g += 2)
+   for (; g + 4 < ngroups; g += 4)
    {
-      group(R1, g, kMore);
-      group(R0, g + 1, kMore);
+      group(B0, g, kMore);
+      group(B1, g + 1, kMore);
+      group(B2, g + 2, kMore);
+      group(B3, g + 3, kMore);
    }
-   if (g + 1 < ngroups)
+   // 1 to 4 groups left, the last of them ends the tile
+   const int left = ngroups - g;
+   if (left == 4)
    {
-      group(R1, g, kMore);
-      group(R0, g + 1, kEnd);
+      group(B0, g, kMore);
+      group(B1, g + 1, kMore);
+      group(B2, g + 2, kMore);
+      group(B3, g + 3, kEnd);
    }
-   else if (g < ngroups)
-      group(R1, g, kEnd);
-   // INLINE_INSERT (a grid that fits the device at once, so nothing waits
-   // for this wave's slot): the wave applies its own survivors -- the region's
-   // entries (its stores are complete before its lanes read them back), or,
-   // when the region overflowed (possible only when very many scores of one
-   // 64 x TILE_N block pass the filter, e.g. near-identical images or minRatio
-   // 0), every score of the block recomputed exactly from the descriptors.
-   // Otherwise k_i8_insert does the same from the region counts.
+   else if (left == 3)
+   {
+      group(B0, g, kMore);
+      group(B1, g + 1, kMore);
+      group(B2, g + 2, kEnd);
+   }
+   else if (left == 2)
+   {
+      group(B0, g, kMore);
+      group(B1, g + 1, kEnd);
+   }
+   else if (left == 1)
+      group(B0, g, kEnd);
    if constexpr (!INLINE_INSERT)
    {
       if (lane == 0) a.survCount[(size_t)tile * kScanWaves + wave] = nsurv;
@@ -377,7 +443,7 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(2)))
       }
    }
    if (!a.persistent) break;
-   __syncthreads(); // every wave is done with this tile's LDS stages
+   __syncthreads(); // every wave is done with this tile's buffers (no load of it is in flight)
    if (tid == 0) sNextTile = (int)nextReq;
    __syncthreads();
    tile = sNextTile;
