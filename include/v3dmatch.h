@@ -243,6 +243,9 @@ int v3dm_batch_info(const v3dm_ctx *ctx, int *nPairs, int64_t *nLeftTotal, int *
 
 /* Scan tile variants: count, and waves per workgroup / tile rows / tile columns. */
 int v3dm_scan_variant_count(void);
+/* Variants 0 .. v3dm_scan_variant_count_f32()-1 also have a float-descriptor
+ * (BF16) form; a float batch accepts only those. */
+int v3dm_scan_variant_count_f32(void);
 int v3dm_scan_variant_info(int variant, int *waves, int *tileRows, int *tileCols);
 
 /* --- synthetic SIFT-like descriptors (host) -------------------------------- */

@@ -78,6 +78,7 @@ _SIGNATURES = {
     "v3dm_batch_set_graph": (ctypes.c_int, [_ctx_p, ctypes.c_int]),
     "v3dm_batch_info": (ctypes.c_int, [_ctx_p, _c_int_p, _i64_p, _c_int_p, _i64_p]),
     "v3dm_scan_variant_count": (ctypes.c_int, []),
+    "v3dm_scan_variant_count_f32": (ctypes.c_int, []),
     "v3dm_set_insert_form": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "v3dm_find_u8_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
                                             ctypes.c_float, ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p]),
@@ -240,6 +241,11 @@ def mfma_layout_probe(device: int = 0) -> np.ndarray:
 def copy_probe(nbytes: int, reps: int = 5, device: int = 0) -> None:
     """Plain device copy of nbytes (read once, written once) per launch."""
     check(lib().v3dm_copy_probe(device, nbytes, reps))
+
+
+def scan_variants_f32() -> int:
+    """Number of scan variants (the first ones) that also run float descriptors."""
+    return int(lib().v3dm_scan_variant_count_f32())
 
 
 def scan_variants() -> list[tuple[int, int, int]]:

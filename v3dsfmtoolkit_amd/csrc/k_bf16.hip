@@ -449,6 +449,9 @@ hipError_t launch_bf16_scan(int variant, int grid, const ScanArgsF &a, hipStream
    return hipGetLastError();
 }
 
+// Tile variants of the BF16 scan: the geometry of int8 scan variants 0-3.
+int bf16_scan_variant_count() { return 4; }
+
 int bf16_scan_resident(int variant, int device)
 {
    const void *fn = nullptr;

@@ -37,6 +37,7 @@
 // key, whose score is the reference's nextVal (a duplicate best counts).
 // Keys of older jobs carry an older epoch and read as empty, so no clearing
 // pass is needed between jobs.
+#include <algorithm>
 #include <atomic>
 #include <type_traits>
 
@@ -647,19 +648,26 @@ static const ScanVariant kVariants[] = {
    {4, 1024, reinterpret_cast<const void *>(&k_i8_scan<4, 1024, false>)},
    {8, 512, reinterpret_cast<const void *>(&k_i8_scan<8, 512, false>)},
    {4, 256, reinterpret_cast<const void *>(&k_i8_scan<4, 256, false>)},
+   {4, 2048, reinterpret_cast<const void *>(&k_i8_scan<4, 2048, false>)},
+   {4, 4096, reinterpret_cast<const void *>(&k_i8_scan<4, 4096, false>)},
 };
+constexpr int kNumVariants = (int)(sizeof(kVariants) / sizeof(kVariants[0]));
 
-int scan_variant_count() { return (int)(sizeof(kVariants) / sizeof(kVariants[0])); }
+int scan_variant_count() { return kNumVariants; }
 int scan_variant_waves(int v) { return kVariants[v].waves; }
 int scan_variant_tile_m(int v) { return kVariants[v].waves * kWaveRows; }
 int scan_variant_tile_n(int v) { return kVariants[v].tileN; }
+// Survivor entries per (tile, wave) region: 64 per 512 right rows, so a
+// region of a wide tile overflows at the same survivor density as one of a
+// 512-wide tile (and the survivor memory per scored pair stays the same).
+int scan_variant_surv_cap(int v) { return 64 * std::max(1, kVariants[v].tileN / 512); }
 
 // Workgroups of variant v resident on the whole device (for persistent launches).
 int scan_variant_resident(int v, int device)
 {
    // asked on every job: remembered per device and variant (0 = not yet known)
-   static std::atomic<int> known[kMaxDevicesCached][4];
-   const bool cacheable = device >= 0 && device < kMaxDevicesCached && v >= 0 && v < 4;
+   static std::atomic<int> known[kMaxDevicesCached][kNumVariants];
+   const bool cacheable = device >= 0 && device < kMaxDevicesCached && v >= 0 && v < kNumVariants;
    if (cacheable)
    {
       const int k = known[device][v].load(std::memory_order_relaxed);
@@ -691,6 +699,10 @@ hipError_t launch_i8_scan(int variant, int grid, const ScanArgs &a, hipStream_t 
    case 5: hipExtLaunchKernelGGL((k_i8_scan<8, 512, true>), dim3(grid), dim3(512), 0, st, start, stop, 0, a); break;
    case 6: hipExtLaunchKernelGGL((k_i8_scan<4, 256, false>), dim3(grid), dim3(256), 0, st, start, stop, 0, a); break;
    case 7: hipExtLaunchKernelGGL((k_i8_scan<4, 256, true>), dim3(grid), dim3(256), 0, st, start, stop, 0, a); break;
+   case 8: hipExtLaunchKernelGGL((k_i8_scan<4, 2048, false>), dim3(grid), dim3(256), 0, st, start, stop, 0, a); break;
+   case 9: hipExtLaunchKernelGGL((k_i8_scan<4, 2048, true>), dim3(grid), dim3(256), 0, st, start, stop, 0, a); break;
+   case 10: hipExtLaunchKernelGGL((k_i8_scan<4, 4096, false>), dim3(grid), dim3(256), 0, st, start, stop, 0, a); break;
+   case 11: hipExtLaunchKernelGGL((k_i8_scan<4, 4096, true>), dim3(grid), dim3(256), 0, st, start, stop, 0, a); break;
    default: return hipErrorInvalidValue;
    }
    return hipGetLastError();

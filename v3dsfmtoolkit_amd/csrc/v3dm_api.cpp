@@ -305,7 +305,7 @@ int enqueueI8Job(v3dm_ctx *c, int v, float minScore, float minRatio, bool marks)
    a.nMasks = (int)c->masks.size();
    a.surv = c->dSurv;
    a.survCount = c->dSurvCount;
-   a.survCap = c->survCap;
+   a.survCap = scan_variant_surv_cap(v);
    // a grid the device holds at once applies its survivors in the scan; a
    // larger one leaves them to k_i8_insert (a wave's tail would hold its slot)
    a.inlineInsert = a.nTiles <= scan_variant_resident(v, c->device) ? 1 : 0;
@@ -746,16 +746,18 @@ int v3dm_allocate(v3dm_ctx *c, int nMaxLeft, int nMaxRight, int useGuided)
    HIPCHK(c, hipMalloc((void **)&c->dColMax, sizeof(int32_t) * nPadL));
    HIPCHK(c, hipMalloc((void **)&c->dRowMax, sizeof(int32_t) * std::max(roundUp(nMaxRight, kRowPad), kRowPad)));
    {
-      // survivor regions for the largest count any scan variant needs
-      int64_t regions = 0;
+      // survivor regions (and entries) for the largest count any scan variant needs
+      int64_t regions = 0, entries = 0;
       for (int v = 0; v < scan_variant_count(); ++v)
       {
          const int64_t tiles =
             (int64_t)((roundUp(std::max(nMaxLeft, 1), kRowPad) + scan_variant_tile_m(v) - 1) / scan_variant_tile_m(v)) *
             ((roundUp(std::max(nMaxRight, 1), kRowPad) + scan_variant_tile_n(v) - 1) / scan_variant_tile_n(v));
          regions = std::max(regions, tiles * scan_variant_waves(v));
+         entries = std::max(entries, tiles * scan_variant_waves(v) *
+                                        std::max<int64_t>(scan_variant_surv_cap(v), c->survCap));
       }
-      HIPCHK(c, hipMalloc((void **)&c->dSurv, sizeof(uint64_t) * (size_t)regions * c->survCap));
+      HIPCHK(c, hipMalloc((void **)&c->dSurv, sizeof(uint64_t) * (size_t)entries));
       HIPCHK(c, hipMalloc((void **)&c->dSurvCount, sizeof(int32_t) * (size_t)regions));
    }
    HIPCHK(c, hipHostMalloc((void **)&c->hMatch, sizeof(int32_t) * nPadL, hipHostMallocDefault));

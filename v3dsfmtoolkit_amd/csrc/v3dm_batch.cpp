@@ -182,7 +182,8 @@ int uploadTable(v3dm_ctx *c, BatchState *b)
    int rc = growDevice(c, b->dSurvCount, b->regionCap, regions);
    if (rc) return rc;
    // grown only when a table needs more: fresh allocations make the next job slow (first touch)
-   rc = growDevice(c, b->dSurv, b->survEntries, std::max<int64_t>(regions, 1) * c->survCap);
+   rc = growDevice(c, b->dSurv, b->survEntries,
+                   std::max<int64_t>(regions, 1) * (b->isFloat ? c->survCap : scan_variant_surv_cap(b->variant)));
    if (rc) return rc;
    b->graphValid = false;
    return syncStream(c);
@@ -220,7 +221,7 @@ void fillArgs(v3dm_ctx *c, BatchState *b, float minScore, float minRatio, ScanAr
    a.nMasks = 0;
    a.surv = b->dSurv;
    a.survCount = b->dSurvCount;
-   a.survCap = c->survCap;
+   a.survCap = scan_variant_surv_cap(v);
    a.waves = scan_variant_waves(v);
    a.tileM = scan_variant_tile_m(v);
    a.tileN = scan_variant_tile_n(v);
@@ -293,6 +294,9 @@ int enqueueKernels(v3dm_ctx *c, BatchState *b, float minScore, float minRatio, b
       fillArgsF(c, b, minScore, minRatio, a, grid);
       // batched jobs run for milliseconds: plain event records around the scan
       int rc;
+      if (b->variant >= bf16_scan_variant_count())
+         return fail(c, V3DM_ERR_UNSUPPORTED, "batch: scan variant %d has no float-descriptor form (0 .. %d)",
+                     b->variant, bf16_scan_variant_count() - 1);
       if (marks && (rc = profMark(c, 0))) return rc;
       HIPCHK(c, launch_bf16_scan(b->variant, grid, a, c->stream));
       if (marks && (rc = profMark(c, 1))) return rc;
@@ -370,7 +374,7 @@ int enqueueJob(v3dm_ctx *c, BatchState *b, float minScore, float minRatio);
 // Times every scan variant on the current batch; returns the fastest.
 int autotune(v3dm_ctx *c, BatchState *b, float minScore, float minRatio, int reps, double *msOut, int *best)
 {
-   const int nv = scan_variant_count();
+   const int nv = b->isFloat ? bf16_scan_variant_count() : scan_variant_count();
    hipEvent_t e0, e1;
    HIPCHK(c, hipEventCreate(&e0));
    HIPCHK(c, hipEventCreate(&e1));
@@ -428,7 +432,8 @@ int pickVariant(v3dm_ctx *c, BatchState *b, float minScore, float minRatio)
    {
       cache = readTuneCache(path);
       auto it = cache.find(key);
-      if (it != cache.end() && it->second >= 0 && it->second < scan_variant_count())
+      if (it != cache.end() && it->second >= 0 &&
+          it->second < (b->isFloat ? bf16_scan_variant_count() : scan_variant_count()))
          return ensureVariant(c, b, it->second);
    }
    int best = 0;
@@ -719,6 +724,7 @@ int v3dm_scan_variant_info(int variant, int *waves, int *tileRows, int *tileCols
 }
 
 int v3dm_scan_variant_count(void) { return scan_variant_count(); }
+int v3dm_scan_variant_count_f32(void) { return bf16_scan_variant_count(); }
 
 int v3dm_batch_autotune(v3dm_ctx *c, float minScore, float minRatio, int reps, double *msPerVariant, int *best)
 {

@@ -20,6 +20,7 @@ int scan_variant_count();
 int scan_variant_waves(int v);
 int scan_variant_tile_m(int v);
 int scan_variant_tile_n(int v);
+int scan_variant_surv_cap(int v);
 int scan_variant_resident(int v, int device);
 int insert_form_for(int64_t regions, int device, int form);
 hipError_t launch_i8_insert(const ScanArgs &a, int form, hipStream_t st);
@@ -27,6 +28,7 @@ hipError_t launch_i8_finalize(const FinalizeArgs &a, hipStream_t st);
 hipError_t launch_bf16_scan(int variant, int grid, const ScanArgsF &a, hipStream_t st, hipEvent_t start = nullptr,
                             hipEvent_t stop = nullptr);
 int bf16_scan_resident(int variant, int device);
+int bf16_scan_variant_count();
 hipError_t launch_f32_rescore(const ScanArgsF &a, hipStream_t st);
 hipError_t launch_f32_finalize(const FinalizeArgs &a, hipStream_t st);
 hipError_t launch_prep_bf16(const float *src, int n, int nPad, uint16_t *dst, uint32_t *norm2Max, hipStream_t st,
@@ -177,7 +179,7 @@ struct v3dm_ctx
    // survivor regions of the fused scan
    uint64_t *dSurv = nullptr;
    int32_t *dSurvCount = nullptr;
-   int survCap = 64;
+   int survCap = 64;        // survivor entries per region of the BF16 scan (int8: per variant)
    int scanVariant = -1;                                 // single-pair int8 scan variant, -1: measured
    int lastVariant = -1;                                 // variant of the last single-pair int8 job
    int lastInline = -1;                                  // its scan applied the survivors itself (1) or not (0)

@@ -590,7 +590,7 @@ int v3dm_learned_find(v3dm_ctx *c, v3dm_mlp *m, int topK, float margin, float mi
          a.epoch = c->dEpoch;
          a.surv = c->dSurv;
          a.survCount = c->dSurvCount;
-         a.survCap = c->survCap;
+         a.survCap = scan_variant_surv_cap(0);
          a.waves = scan_variant_waves(0);
          a.tileM = scan_variant_tile_m(0);
          a.tileN = scan_variant_tile_n(0);
