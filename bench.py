@@ -96,6 +96,8 @@ def main():
                     help="single-pair int8 scan variant (-1: the library measures the fastest)")
     ap.add_argument("--profile-variants", action="store_true",
                     help="after the measurement, run 20 jobs with each scan variant (for profiler passes)")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="c2 only: skip the other workloads' summaries (batch, learned path, 65536^2)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -164,10 +166,40 @@ def run_c2(args, v):
     t0 = time.perf_counter()
     for _ in range(args.steps):
         m.findAsync(min_score, min_ratio)
+    t_enq = time.perf_counter()
     m.synchronize()
     t1 = time.perf_counter()
     final = m.fetchMatches()
     assert final == got
+    host_enqueue_us = (t_enq - t0) / args.steps * 1e6
+    # what a reference caller pays per pair (match_sequence.cpp:184-203): host
+    # descriptors uploaded, the job, the list back, every call blocking
+    # (the C-ABI calls themselves, as a C++ caller makes them: no Python list building)
+    import ctypes
+    lib, ctx = m._lib, m._ctx
+    pbuf = np.empty((n, 2), np.int32)
+    cnt = ctypes.c_int(0)
+    sync = {}
+    for kind, (hl, hr) in {"u8": (L, R), "f32": (L.astype(np.float32) / 512, R.astype(np.float32) / 512)}.items():
+        ct = ctypes.c_uint8 if kind == "u8" else ctypes.c_float
+        setl = lib.v3dm_set_left_u8 if kind == "u8" else lib.v3dm_set_left_f32
+        setr = lib.v3dm_set_right_u8 if kind == "u8" else lib.v3dm_set_right_f32
+        pl, pr = hl.ctypes.data_as(ctypes.POINTER(ct)), hr.ctypes.data_as(ctypes.POINTER(ct))
+        lat = []
+        for _ in range(12):
+            ts = time.perf_counter()
+            rc = setl(ctx, n, pl) or setr(ctx, n, pr) or lib.v3dm_find_putative_matches(
+                ctx, min_score, min_ratio, pbuf.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), n,
+                ctypes.byref(cnt))
+            lat.append(time.perf_counter() - ts)
+            assert rc == 0
+        assert list(map(tuple, pbuf[:cnt.value].tolist())) == got
+        sync[kind] = {"median_ms": statistics.median(lat[2:]) * 1e3, "min_ms": min(lat[2:]) * 1e3,
+                      "upload_bytes": int(hl.nbytes + hr.nbytes)}
+    sync["what"] = ("v3dm_set_left / v3dm_set_right from pageable host arrays + v3dm_find_putative_matches "
+                    "(pairs into a host array), blocking, per pair: what a reference caller of "
+                    "setLeftFeatures / setRightFeatures / findPutativeMatches pays; f32: the floats u/512 the "
+                    "reference callers pass")
 
     wall = (t1 - t0) / args.steps
     pairs = n * n
@@ -178,9 +210,9 @@ def run_c2(args, v):
         scan_kernel = _scan_name(v, m.lastScanVariant(), m.lastScanInline() == 1)
         roof = {"bound": "mfma", "kernel": scan_kernel, "achieved": achieved,
                 "peak": INT8_PEAK_TOPS,
-                "unit": "TFLOP/s", "frac": achieved / INT8_PEAK_TOPS,
+                "unit": "TOP/s", "frac": achieved / INT8_PEAK_TOPS,
                 "traffic": measured_traffic(scan_kernel) if n == 8192 else None,
-                "ops_per_pair": 256, "launch_avg_us": scan_avg_s * 1e6,
+                "ops_per_pair": 256, "launch_avg_us": scan_avg_s * 1e6, "peak_unit": "int8 dense MFMA, TOP/s",
                 # the same work over the whole step (every kernel of the job, launch gaps included)
                 "job_frac": pairs * 256 / wall / 1e12 / INT8_PEAK_TOPS}
         if m.lastScanInline() == 1:
@@ -212,6 +244,8 @@ def run_c2(args, v):
                                     if kernel == v.KERNEL_I8 else None)},
         "image_pairs_per_s": 1.0 / wall,
         "select_avg_us": sel_avg_s * 1e6,
+        "host_enqueue_us_per_step": host_enqueue_us,
+        "sync_find_putative_matches": sync,
         "roofline": roof,
     }
     if args.plain_steps > 0:
@@ -234,7 +268,50 @@ def run_c2(args, v):
     m.close()
     if args.copy_probe_mib > 0:
         copy_probe(args.copy_probe_mib)
+    if not args.no_extra and n == 8192 and kernel == v.KERNEL_I8:
+        out["other_workloads"] = run_extra(args, v)
     print(json.dumps(out), flush=True)
+
+
+def run_extra(args, v):
+    """The other workloads of the metric, each summarised (full lines:
+    --workload c3 / c4 / c5): the 256-pair batch (image pairs/s, scan roofline,
+    float variant), the learned path on 64 pairs of 16384^2 (recall,
+    precision, fallback, speed against the plain and the exact int8 path), and
+    the 65536^2 pair (scan roofline, top-32 learned recall)."""
+    sub = argparse.Namespace(**vars(args))
+    sub.pairs = 0
+    res = {}
+    t = time.perf_counter()
+    c3 = run_c3(sub, v)
+    res["batch_256_pairs"] = {
+        "image_pairs_per_s": c3["image_pairs_per_s"], "descriptor_pairs_per_s": c3["value"],
+        "ms_per_job": c3["ms_per_step"], "roofline": c3["roofline"], "config": c3["config"],
+        "float_variant": {k: c3["float_variant"][k] for k in ("image_pairs_per_s", "ms_per_job",
+                                                               "sample_equals_fp32_oracle",
+                                                               "sample_max_abs_score_error_vs_fp64",
+                                                               "sample_list_disagreement_vs_fp64_rule",
+                                                               "sample_fp64_rule_matches")},
+        "sample_equals_oracle": True, "seconds": time.perf_counter() - t}
+    progress("batch summary done")
+    t = time.perf_counter()
+    c4 = run_c4(sub, v)
+    res["learned_64_pairs_16384"] = {k: c4[k] for k in ("recall", "precision", "exact_matches",
+                                                        "fallback_fraction_left", "fallback_fraction_right",
+                                                        "ms_per_pair", "speedup_vs_plain", "training")}
+    res["learned_64_pairs_16384"]["speedup_vs_exact_int8"] = (c4["ms_per_pair"]["exact_int8"] /
+                                                             c4["ms_per_pair"]["learned"])
+    res["learned_64_pairs_16384"]["seconds"] = time.perf_counter() - t
+    progress("learned summary done")
+    t = time.perf_counter()
+    c5 = run_c5(sub, v)
+    res["pair_65536"] = {"descriptor_pairs_per_s": c5["value"], "ms_per_job": c5["ms_per_step"],
+                         "roofline": c5["roofline"], "matches": c5["config"]["matches"],
+                         "learned_top32": {k: c5["learned_top32"][k] for k in ("ms_per_pair", "recall", "precision",
+                                                                                "fallback_fraction_left")},
+                         "plain": c5["plain"], "seconds": time.perf_counter() - t}
+    progress("65536^2 summary done")
+    return res
 
 
 def _scan_name(v, variant, inline=False):
@@ -326,8 +403,9 @@ def run_c3(args, v):
                    "tile": [tm, tn], "waves": w, "graph": True})
     out["image_pairs_per_s"] = n_pairs / wall
     out["autotune_ms"] = [float(x) for x in tune_ms]
-    out["roofline"] = {"bound": "mfma", "kernel": "k_i8_scan (batched)", "achieved": achieved,
-                       "peak": INT8_PEAK_TOPS, "unit": "TFLOP/s", "frac": achieved / INT8_PEAK_TOPS, "traffic": None,
+    out["roofline"] = {"bound": "mfma", "kernel": _scan_name(v, best, False) + " (persistent, batched)",
+                       "achieved": achieved,
+                       "peak": INT8_PEAK_TOPS, "unit": "TOP/s", "frac": achieved / INT8_PEAK_TOPS, "traffic": None,
                        "launch_avg_us": scan_avg * 1e6}
     out["cpu_baseline"] = {"value": cpu_pairs / cpu_t, "unit": "descriptor pairs scored/s", "cores": threads,
                            "kind": "port", "sample": f"{len(sample)} of the pairs ({cpu_pairs} descriptor pairs), "
@@ -508,6 +586,7 @@ def run_c5(args, v):
     dt_i8, _ = _time_job(lambda: (m.findAsync(0.8, 0.8), m.synchronize()), 10)
     scan_ms, sel_ms, jobs = m.profileRead()
     m.profileEnable(False)
+    scan_var, scan_inline = m.lastScanVariant(), m.lastScanInline() == 1
     scan_avg = scan_ms / 1e3 / max(jobs, 1)
     net, train = _train_net(v, 1_000_000, 0.3, 0.08, steps=1500)
     dt_l, (got, st) = _time_job(lambda: m.learnedFind(net, 0.05, 0.8, 0.8, 32, asArray=True), 3)
@@ -527,8 +606,9 @@ def run_c5(args, v):
     out = _common("descriptor pairs scored/s", n * n / dt_i8, "descriptor pairs/s", 10, 1, dt_i8, "int8",
                   {"workload": f"single pair {n}x{n} uint8, 30% noisy copies (sigma 0.08), minScore 0.8, "
                                "minRatio 0.8, mutual", "n_left": n, "n_right": n, "matches": len(exact)})
-    out["roofline"] = {"bound": "mfma", "kernel": "k_i8_scan", "achieved": achieved, "peak": INT8_PEAK_TOPS,
-                       "unit": "TFLOP/s", "frac": achieved / INT8_PEAK_TOPS, "traffic": None,
+    out["roofline"] = {"bound": "mfma", "kernel": _scan_name(v, scan_var, scan_inline), "achieved": achieved,
+                       "peak": INT8_PEAK_TOPS,
+                       "unit": "TOP/s", "frac": achieved / INT8_PEAK_TOPS, "traffic": None,
                        "launch_avg_us": scan_avg * 1e6}
     out["learned_top32"] = {"ms_per_pair": dt_l * 1e3, "effective_pairs_per_s": n * n / dt_l,
                             "recall": hit / max(len(e), 1), "precision": hit / max(len(g), 1),

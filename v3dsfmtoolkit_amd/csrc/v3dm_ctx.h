@@ -123,6 +123,7 @@ struct Side
    int32_t *dSum = nullptr;
    uint64_t *dTop1 = nullptr, *dTop2 = nullptr;
    float *dPos = nullptr; // kMaxMasks position sets (guided)
+   void *hStage = nullptr;   // pinned host staging of the caller's descriptors (nMax x 128 floats)
    uint16_t *dBF = nullptr; // BF16 pool (float path), rows padded to kRowPad
    bool bfValid = false;
    float norm2Max = 0.0f;   // largest squared row norm (BF16 error bound)

@@ -291,7 +291,7 @@ class SIFTMatching:
 
 
 def _as_list(pairs: np.ndarray, n: int) -> list[tuple[int, int]]:
-    return [(int(a), int(b)) for a, b in pairs[:n]]
+    return list(map(tuple, pairs[:n].tolist()))
 
 
 class EmbeddingNet:
