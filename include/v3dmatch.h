@@ -292,6 +292,15 @@ const char *v3dm_mlp_last_error(const v3dm_mlp *net);
 int v3dm_mlp_get_params(v3dm_mlp *net, float *params);
 /* Sets the FP32 master parameters and resets the optimiser state. */
 int v3dm_mlp_set_params(v3dm_mlp *net, const float *params);
+/* Saves the network as inference uses it: an 8-byte tag "V3DMMLP\0", six
+ * little-endian uint32 (version 1, widths 128 / 256 / 64, parameter count,
+ * FNV-1a checksum of the payload), then W1 and W2 as BF16 and b1, b2 as FP32
+ * in the parameter layout (99584 bytes).  v3dm_mlp_load checks all of it and
+ * sets the masters to exactly those values (a loaded network embeds bit for
+ * bit like the saved one; optimiser state reset).  No counterpart in the
+ * reference. */
+int v3dm_mlp_save(v3dm_mlp *net, const char *path);
+int v3dm_mlp_load(v3dm_mlp *net, const char *path);
 /* Loss and parameter gradient of one batch of matched pairs (left[i], right[i]),
  * batch x 128 uint8 each, at the given temperature; no update. */
 int v3dm_mlp_loss_grad(v3dm_mlp *net, int batch, const uint8_t *left, const uint8_t *right, float temperature,

@@ -113,3 +113,85 @@ def test_trained_network_beats_untrained(m, trained):
     assert out["trained"][2] >= out["untrained"][2]
     # trained on sigma 0.05 pairs only, evaluated at sigma 0.08
     assert out["trained"][0] > 0.95 and out["trained"][1] > 0.995
+
+
+def test_saved_network_loads_bit_for_bit(m, trained, tmp_path):
+    """Persistence (BF16 weights + FP32 biases with a checked header): a
+    network loaded from the file embeds exactly like the saved one and leads
+    to the same match list; truncated or altered files are refused."""
+    say("save / load")
+    path = tmp_path / "net.v3dmlp"
+    trained.save(path)
+    assert path.stat().st_size == 32 + 99584
+    d = eval_pair(5, n=2048)
+    other = v.EmbeddingNet(0, seed=99)
+    try:
+        other.load(path)
+        np.testing.assert_array_equal(other.embed(d["L"]).view(np.uint32), trained.embed(d["L"]).view(np.uint32))
+        m.setLeftFeatures(2048, d["L"])
+        m.setRightFeatures(2048, d["R"])
+        a, _ = m.learnedFind(trained, 0.05, 0.8, 0.8)
+        b, _ = m.learnedFind(other, 0.05, 0.8, 0.8)
+        assert a == b
+        raw = bytearray(path.read_bytes())
+        raw[5000] ^= 0x10
+        bad = tmp_path / "bad.v3dmlp"
+        bad.write_bytes(bytes(raw))
+        with pytest.raises(v.V3DMError, match="checksum"):
+            other.load(bad)
+        bad.write_bytes(bytes(raw[:-4]))
+        with pytest.raises(v.V3DMError, match="not a saved network"):
+            other.load(bad)
+    finally:
+        other.close()
+
+
+@pytest.fixture(scope="module")
+def trained_noisy(gpu_available):
+    """A network trained on exact-labelled pairs of the sigma 0.08 generator."""
+    from v3dsfmtoolkit_amd import learned as VL
+    L, R = VL.labeled_pairs(200000, n=8192, rho=0.3, sigma=0.08)
+    net = v.EmbeddingNet(0, seed=5)
+    VL.train(net, L, R, steps=600, batch=1024, lr=2e-3)
+    yield net
+    net.close()
+
+
+def test_trained_network_beats_a_random_projection(m, trained_noisy):
+    """The ablation: the same pipeline (top-16 retrieval in 64-D, exact
+    rescoring, margin fallback) with a random orthogonal 128 -> 64 projection
+    in place of the trained network, on noisier copies (sigma 0.10).  Training
+    must rank each noisy copy's source higher (first candidate, and among the
+    16); the pipeline's recall of the exact match set stays >= 0.995 with
+    either, since the exhaustive fallback catches what retrieval misses
+    (fallback shares and times are printed)."""
+    import time
+    from v3dsfmtoolkit_amd.learned import projection_params
+    say("trained vs random orthogonal projection")
+    d = eval_pair(8, rho=0.3, sigma=0.10)
+    exact = O.find_u8(d["L"], d["R"], 0.8, 0.8)[2]
+    src = d["src"]
+    rights = np.flatnonzero(src >= 0)
+    lefts = src[rights]
+    proj = v.EmbeddingNet(0, seed=1)
+    proj.set_params(projection_params(3))
+    out = {}
+    try:
+        for name, net in [("projection", proj), ("trained", trained_noisy)]:
+            m.setLeftFeatures(4096, d["L"])
+            m.setRightFeatures(4096, d["R"])
+            t = time.perf_counter()
+            got, st = m.learnedFind(net, 0.05, 0.8, 0.8)
+            dt = time.perf_counter() - t
+            cand = m.learnedCandidates()
+            copy1 = np.mean(cand[lefts, 0] == rights)
+            copy16 = np.mean([rights[i] in cand[lefts[i]] for i in range(len(rights))])
+            rec, prec = recall_precision(got, exact)
+            out[name] = (copy1, copy16, rec)
+            say(f"{name}: source of a copy first {copy1:.4f}, among 16 {copy16:.4f}; pipeline recall {rec:.4f} "
+                f"precision {prec:.4f}, fallback {st['flagged_left'] / 4096:.3f} of lefts, {dt * 1e3:.2f} ms")
+    finally:
+        proj.close()
+    tr, pj = out["trained"], out["projection"]
+    assert tr[0] > pj[0] + 0.02 and tr[1] > pj[1]
+    assert tr[2] >= 0.995 and pj[2] >= 0.995

@@ -88,3 +88,21 @@ def test_learned_oracle_candidates_matter():
     tab, _, _ = OL.learned_match(d["L"], d["R"], cand, -1.0, 0.8, 0.8)
     want = O.find_u8(d["L"], d["R"], 0.8, 0.8)[2]
     assert (tab >= 0).sum() < (want >= 0).sum() / 2
+
+
+def test_projection_params_make_the_network_a_projection():
+    """The ablation's parameters (v3dsfmtoolkit_amd.learned.projection_params):
+    through the network oracle's float64 forward pass the embedding equals
+    P x / |P x| with the BF16-rounded P, since GELU(a) - GELU(-a) = a; and P
+    has orthonormal rows before rounding."""
+    from v3dsfmtoolkit_amd.learned import projection_params
+    p = projection_params(3)
+    W1 = p[:256 * 128].reshape(256, 128)
+    P = W1[:64].astype(np.float64)
+    np.testing.assert_allclose(P @ P.T, np.eye(64), atol=1e-6)
+    U = np.random.default_rng(4).integers(0, 256, size=(200, 128)).astype(np.uint8)
+    e = OM.embed(p, U)
+    Pr = OM.bf16_round(W1[:64]).astype(np.float64)
+    z = (U.astype(np.float64) / 64.0) @ Pr.T
+    want = z / np.linalg.norm(z, axis=1, keepdims=True)
+    np.testing.assert_allclose(e, want, atol=1e-12)

@@ -100,6 +100,8 @@ _SIGNATURES = {
     "v3dm_mlp_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
     "v3dm_mlp_get_params": (ctypes.c_int, [ctypes.c_void_p, _f32_p]),
     "v3dm_mlp_set_params": (ctypes.c_int, [ctypes.c_void_p, _f32_p]),
+    "v3dm_mlp_save": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
+    "v3dm_mlp_load": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
     "v3dm_mlp_loss_grad": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _u8_p, _u8_p, ctypes.c_float, _f32_p,
                                           _f32_p]),
     "v3dm_mlp_set_training_pairs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, _u8_p, _u8_p]),

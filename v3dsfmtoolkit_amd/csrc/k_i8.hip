@@ -211,6 +211,7 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(2)))
    if (a.persistent && tid == 0) nextReq = atomicAdd(a.tileCounter, 1u) + gridDim.x;
    PairDesc pd = a.pair0; // single pair: straight from the kernel arguments
    if (a.nPairs > 1) pd = a.pairs[sgpr(pair_of_tile(a.tilePair, a.pairs, a.nPairs, tile))];
+   else if (a.pairsOnDevice) pd = a.pairs[0];
    // tile parameters are workgroup-uniform: keep them in scalar registers
    pd.offL = ((int64_t)sgpr((int)(pd.offL >> 32)) << 32) | (uint32_t)sgpr((int)pd.offL);
    pd.offR = ((int64_t)sgpr((int)(pd.offR >> 32)) << 32) | (uint32_t)sgpr((int)pd.offR);
@@ -221,6 +222,7 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(2)))
    pd.tilesN = sgpr(pd.tilesN);
    pd.tileBase = sgpr(pd.tileBase);
    const int t = tile - pd.tileBase;
+   if (a.pairsOnDevice && t >= sgpr(pd.tilesM) * pd.tilesN) break; // past the job's own grid (never persistent)
    const int tm = t / pd.tilesN, tn = t - tm * pd.tilesN;
    const int nLpad = (pd.nL + kRowPad - 1) / kRowPad * kRowPad;
    const int nRpad = (pd.nR + kRowPad - 1) / kRowPad * kRowPad;
@@ -485,6 +487,7 @@ __global__ __launch_bounds__(256) void k_i8_insert(ScanArgs a)
       wFirst = wid - tile * a.waves;
       wCount = 1;
    }
+   if (a.pairsOnDevice && tile >= a.pairs[0].tilesM * a.pairs[0].tilesN) return; // no scan tile wrote it
    const size_t r0 = (size_t)tile * a.waves + wFirst;
    int cnt[kSpan];
    uint64_t w0[kSpan];
@@ -505,7 +508,7 @@ __global__ __launch_bounds__(256) void k_i8_insert(ScanArgs a)
 #pragma unroll
    for (int w = 0; w < kSpan; ++w) any |= cnt[w];
    if (any == 0) return;
-   const PairDesc pd = a.nPairs > 1 ? a.pairs[p] : a.pair0; // a single pair comes by value
+   const PairDesc pd = (a.nPairs > 1 || a.pairsOnDevice) ? a.pairs[p] : a.pair0; // else the pair comes by value
    const int t = tile - pd.tileBase;
    const int tm = t / pd.tilesN, tn = t - tm * pd.tilesN;
    const int c0 = tn * a.tileN;

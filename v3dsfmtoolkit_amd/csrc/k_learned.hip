@@ -431,10 +431,19 @@ __global__ void k_flag(int n, int isRight, const uint64_t *__restrict__ top1, co
 
 // Gathers rows idx[0..n) of an int8 pool and its row sums into a compact
 // pool of nPad rows (rows >= n zero, with the padding row sum).
-__global__ void k_gather_rows(int n, int nPad, const int *__restrict__ idx, const int8_t *__restrict__ src,
-                              const int32_t *__restrict__ srcSum, int8_t *__restrict__ dst, int32_t *__restrict__ dstSum)
+// nDev (may be null): the row count in device memory (a flag count of this
+// job); the grid then covers the largest count and the rows past the count's
+// own 128-row padding are left alone.
+__global__ void k_gather_rows(int n, int nPad, const int *__restrict__ nDev, const int *__restrict__ idx,
+                              const int8_t *__restrict__ src, const int32_t *__restrict__ srcSum,
+                              int8_t *__restrict__ dst, int32_t *__restrict__ dstSum)
 {
    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+   if (nDev)
+   {
+      n = *nDev;
+      nPad = max((n + kRowPad - 1) / kRowPad * kRowPad, kRowPad);
+   }
    if (t >= (int64_t)nPad * (kDim / 16)) return;
    const int row = (int)(t / (kDim / 16)), part = (int)(t % (kDim / 16));
    uint4 v = make_uint4(0, 0, 0, 0);
@@ -519,13 +528,37 @@ hipError_t launch_flag(int n, int isRight, const uint64_t *top1, const uint64_t 
    return hipGetLastError();
 }
 
-hipError_t launch_gather_rows(int n, int nPad, const int *idx, const int8_t *src, const int32_t *srcSum, int8_t *dst,
-                              int32_t *dstSum, hipStream_t st)
+hipError_t launch_gather_rows(int n, int nPad, const int *nDev, const int *idx, const int8_t *src,
+                              const int32_t *srcSum, int8_t *dst, int32_t *dstSum, hipStream_t st)
 {
    const int64_t t = (int64_t)nPad * (kDim / 16);
    if (t <= 0) return hipSuccess;
-   hipLaunchKernelGGL(k_gather_rows, dim3((unsigned)((t + 255) / 256)), dim3(256), 0, st, n, nPad, idx, src, srcSum,
-                      dst, dstSum);
+   hipLaunchKernelGGL(k_gather_rows, dim3((unsigned)((t + 255) / 256)), dim3(256), 0, st, n, nPad, nDev, idx, src,
+                      srcSum, dst, dstSum);
+   return hipGetLastError();
+}
+
+// The two fallback scans' pair descriptors from this job's flag counts:
+// side 0 = the flagged lefts against every right, side 1 = every left
+// against the flagged rights, tiles of tileM x tileN.
+__global__ void k_fallback_pairs(const int *__restrict__ count, int nL, int nR, int tileM, int tileN,
+                                 PairDesc *__restrict__ out)
+{
+   if (threadIdx.x >= 2) return;
+   const int side = threadIdx.x;
+   PairDesc q{};
+   q.nL = side == 0 ? count[0] : nL;
+   q.nR = side == 0 ? nR : count[1];
+   q.tilesM = ((max(q.nL, 1) + kRowPad - 1) / kRowPad * kRowPad + tileM - 1) / tileM;
+   q.tilesN = ((max(q.nR, 1) + kRowPad - 1) / kRowPad * kRowPad + tileN - 1) / tileN;
+   if (q.nL == 0 || q.nR == 0) q.tilesM = q.tilesN = 0;
+   out[side] = q;
+}
+
+hipError_t launch_fallback_pairs(const int *count, int nL, int nR, int tileM, int tileN, PairDesc *out,
+                                 hipStream_t st)
+{
+   hipLaunchKernelGGL(k_fallback_pairs, dim3(1), dim3(64), 0, st, count, nL, nR, tileM, tileN, out);
    return hipGetLastError();
 }
 

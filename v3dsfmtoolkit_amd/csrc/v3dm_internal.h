@@ -99,6 +99,8 @@ struct ScanArgs
    uint32_t *tileCounter;       // reset to 0 by the finalize kernel
    const int32_t *mapL, *mapR;  // optional: feature index of each row of a gathered pool
    const int32_t *tilePair;     // optional: index into pairs of every tile (else a binary search)
+   int pairsOnDevice;           // 1: the one pair's descriptor is pairs[0], written on the device by an
+                                // earlier kernel of the job; tiles past its tile count exit at once
 };
 
 // Float descriptors: BF16 MFMA candidate scan + exact FP32 rescoring.

@@ -11,6 +11,8 @@
 #include <cmath>
 #include <random>
 
+#include <vector>
+
 #include "v3dm_ctx.h"
 
 namespace v3dm {
@@ -39,8 +41,10 @@ hipError_t launch_cand_insert(int nL, int nR, int K, const int *cand, const int8
                               const uint32_t *epoch, hipStream_t st);
 hipError_t launch_flag(int n, int isRight, const uint64_t *top1, const uint64_t *top2, const uint32_t *epoch,
                        float margin, const int *force, int *list, int *count, hipStream_t st);
-hipError_t launch_gather_rows(int n, int nPad, const int *idx, const int8_t *src, const int32_t *srcSum, int8_t *dst,
-                              int32_t *dstSum, hipStream_t st);
+hipError_t launch_gather_rows(int n, int nPad, const int *nDev, const int *idx, const int8_t *src,
+                              const int32_t *srcSum, int8_t *dst, int32_t *dstSum, hipStream_t st);
+hipError_t launch_fallback_pairs(const int *count, int nL, int nR, int tileM, int tileN, PairDesc *out,
+                                 hipStream_t st);
 } // namespace v3dm
 
 namespace {
@@ -354,6 +358,122 @@ int v3dm_mlp_get_params(v3dm_mlp *m, float *params)
    return V3DM_OK;
 }
 
+// ---- persistence -------------------------------------------------------------
+// File: "V3DMMLP" + NUL, then little-endian uint32 version (1), input, hidden
+// and output widths, parameter count, FNV-1a checksum of the payload; the
+// payload is what inference uses, in the parameter layout: W1 as BF16,
+// b1 as FP32, W2 as BF16, b2 as FP32 (99584 bytes).  Loading sets the FP32
+// masters to exactly those values, so a loaded network embeds bit for bit
+// like the saved one (training resumes from BF16-valued weights).
+
+namespace {
+constexpr char kMlpMagic[8] = {'V', '3', 'D', 'M', 'M', 'L', 'P', 0};
+constexpr uint32_t kMlpVersion = 1;
+
+uint16_t bf16_bits(float f) // round to nearest even, as the forward copy is rounded on the device
+{
+   uint32_t u;
+   memcpy(&u, &f, 4);
+   return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+float bf16_value(uint16_t b)
+{
+   const uint32_t u = (uint32_t)b << 16;
+   float f;
+   memcpy(&f, &u, 4);
+   return f;
+}
+uint32_t fnv1a(const std::vector<uint8_t> &v)
+{
+   uint32_t h = 2166136261u;
+   for (uint8_t x : v) h = (h ^ x) * 16777619u;
+   return h;
+}
+bool isWeight(int i) { return i < kOffB1 || (i >= kOffW2 && i < kOffB2); }
+} // namespace
+
+int v3dm_mlp_save(v3dm_mlp *m, const char *path)
+{
+   if (!m || !path) return V3DM_ERR_ARG;
+   std::vector<float> p(kParams);
+   int rc = v3dm_mlp_get_params(m, p.data());
+   if (rc) return rc;
+   std::vector<uint8_t> payload;
+   payload.reserve(2 * (kOffB1 + kOut * kHid) + 4 * (kHid + kOut));
+   for (int i = 0; i < kParams; ++i)
+   {
+      if (isWeight(i))
+      {
+         const uint16_t b = bf16_bits(p[i]);
+         payload.push_back((uint8_t)(b & 0xFF));
+         payload.push_back((uint8_t)(b >> 8));
+      }
+      else
+      {
+         uint32_t u;
+         memcpy(&u, &p[i], 4);
+         for (int k = 0; k < 4; ++k) payload.push_back((uint8_t)(u >> (8 * k)));
+      }
+   }
+   const uint32_t head[6] = {kMlpVersion, (uint32_t)kIn, (uint32_t)kHid, (uint32_t)kOut, (uint32_t)kParams,
+                             fnv1a(payload)};
+   FILE *f = fopen(path, "wb");
+   if (!f) return mfail(m, V3DM_ERR_ARG, "mlp_save: cannot open %s for writing", path);
+   bool ok = fwrite(kMlpMagic, 1, 8, f) == 8;
+   for (uint32_t h : head)
+   {
+      const uint8_t le[4] = {(uint8_t)h, (uint8_t)(h >> 8), (uint8_t)(h >> 16), (uint8_t)(h >> 24)};
+      ok = ok && fwrite(le, 1, 4, f) == 4;
+   }
+   ok = ok && fwrite(payload.data(), 1, payload.size(), f) == payload.size();
+   ok = (fclose(f) == 0) && ok;
+   if (!ok) return mfail(m, V3DM_ERR_ARG, "mlp_save: write to %s failed", path);
+   return V3DM_OK;
+}
+
+int v3dm_mlp_load(v3dm_mlp *m, const char *path)
+{
+   if (!m || !path) return V3DM_ERR_ARG;
+   FILE *f = fopen(path, "rb");
+   if (!f) return mfail(m, V3DM_ERR_ARG, "mlp_load: cannot open %s", path);
+   std::vector<uint8_t> all;
+   uint8_t buf[65536];
+   size_t got;
+   while ((got = fread(buf, 1, sizeof(buf), f)) > 0) all.insert(all.end(), buf, buf + got);
+   fclose(f);
+   const size_t payloadBytes = 2 * (size_t)(kOffB1 + kOut * kHid) + 4 * (size_t)(kHid + kOut);
+   if (all.size() != 8 + 24 + payloadBytes || memcmp(all.data(), kMlpMagic, 8) != 0)
+      return mfail(m, V3DM_ERR_ARG, "mlp_load: %s is not a saved network (%zu bytes)", path, all.size());
+   uint32_t head[6];
+   for (int k = 0; k < 6; ++k)
+      head[k] = (uint32_t)all[8 + 4 * k] | ((uint32_t)all[9 + 4 * k] << 8) | ((uint32_t)all[10 + 4 * k] << 16) |
+                ((uint32_t)all[11 + 4 * k] << 24);
+   if (head[0] != kMlpVersion || head[1] != (uint32_t)kIn || head[2] != (uint32_t)kHid || head[3] != (uint32_t)kOut ||
+       head[4] != (uint32_t)kParams)
+      return mfail(m, V3DM_ERR_UNSUPPORTED, "mlp_load: %s: version %u, shape %u-%u-%u, %u parameters", path, head[0],
+                   head[1], head[2], head[3], head[4]);
+   const std::vector<uint8_t> payload(all.begin() + 32, all.end());
+   if (fnv1a(payload) != head[5]) return mfail(m, V3DM_ERR_ARG, "mlp_load: %s: checksum mismatch", path);
+   std::vector<float> p(kParams);
+   size_t o = 0;
+   for (int i = 0; i < kParams; ++i)
+   {
+      if (isWeight(i))
+      {
+         p[i] = bf16_value((uint16_t)(payload[o] | (payload[o + 1] << 8)));
+         o += 2;
+      }
+      else
+      {
+         const uint32_t u = (uint32_t)payload[o] | ((uint32_t)payload[o + 1] << 8) | ((uint32_t)payload[o + 2] << 16) |
+                            ((uint32_t)payload[o + 3] << 24);
+         memcpy(&p[i], &u, 4);
+         o += 4;
+      }
+   }
+   return v3dm_mlp_set_params(m, p.data());
+}
+
 int v3dm_mlp_set_params(v3dm_mlp *m, const float *params)
 {
    if (!m || !params) return V3DM_ERR_ARG;
@@ -547,41 +667,35 @@ int v3dm_learned_find(v3dm_ctx *c, v3dm_mlp *m, int topK, float margin, float mi
       HIPCHK(c, launch_flag(nL, 0, c->L.dTop1, c->L.dTop2, c->dEpoch, margin, s->ovf, s->flagL, s->dCount, st));
       HIPCHK(c, launch_flag(nR, 1, c->R.dTop1, c->R.dTop2, c->dEpoch, margin, nullptr, s->flagR, s->dCount + 1,
                             st));
-      HIPCHK(c, hipMemcpyAsync(s->hCount, s->dCount, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
-      rc = syncStream(c);
-      if (rc) return rc;
-      nFlag[0] = s->hCount[0];
-      nFlag[1] = s->hCount[1];
-      // 5. exhaustive int8 scans of the flagged lefts (all rights) and flagged rights (all lefts)
-      PairDesc fb[2];
-      for (int side = 0; side < 2; ++side)
-      {
-         const int nf = (int)nFlag[side];
-         PairDesc &q = fb[side];
-         q = PairDesc{};
-         q.nL = side == 0 ? nf : nL;
-         q.nR = side == 0 ? nR : nf;
-         q.tilesM = (roundUp(std::max(q.nL, 1), kRowPad) + scan_variant_tile_m(0) - 1) / scan_variant_tile_m(0);
-         q.tilesN = (roundUp(std::max(q.nR, 1), kRowPad) + scan_variant_tile_n(0) - 1) / scan_variant_tile_n(0);
-      }
-      HIPCHK(c, hipMemcpyAsync(s->dPairFb, fb, sizeof(fb), hipMemcpyHostToDevice, st));
-      for (int side = 0; side < 2; ++side)
-      {
-         const int nf = (int)nFlag[side];
-         if (nf == 0) continue;
-         const int nPad = std::max(roundUp(nf, kRowPad), kRowPad);
-         const Side &src = side == 0 ? c->L : c->R;
-         HIPCHK(c, launch_gather_rows(nf, nPad, side == 0 ? s->flagL : s->flagR, src.dI8, src.dSum, s->gI8, s->gSum,
+      // 5. exhaustive int8 scans of the flagged lefts (all rights) and flagged
+      // rights (all lefts), with no host round trip: the flag counts stay on
+      // the device, the gather and the scans are launched for the largest
+      // count (every row flagged) and read the real one -- the scans from
+      // pair descriptors a kernel writes from the counts
+      HIPCHK(c, launch_fallback_pairs(s->dCount, nL, nR, scan_variant_tile_m(0), scan_variant_tile_n(0), s->dPairFb,
                                       st));
+      for (int side = 0; side < 2; ++side)
+      {
+         const int nMax = side == 0 ? nL : nR;
+         const int nPad = std::max(roundUp(nMax, kRowPad), kRowPad);
+         const Side &src = side == 0 ? c->L : c->R;
+         HIPCHK(c, launch_gather_rows(nMax, nPad, s->dCount + side, side == 0 ? s->flagL : s->flagR, src.dI8, src.dSum,
+                                      s->gI8, s->gSum, st));
+         PairDesc worst{};
+         worst.nL = nL;
+         worst.nR = nR;
+         worst.tilesM = (roundUp(nL, kRowPad) + scan_variant_tile_m(0) - 1) / scan_variant_tile_m(0);
+         worst.tilesN = (roundUp(nR, kRowPad) + scan_variant_tile_n(0) - 1) / scan_variant_tile_n(0);
          ScanArgs a{};
-         a.pair0 = fb[side];
+         a.pair0 = worst; // not read: pairsOnDevice
          a.L = side == 0 ? s->gI8 : c->L.dI8;
          a.Lsum = side == 0 ? s->gSum : c->L.dSum;
          a.R = side == 0 ? c->R.dI8 : s->gI8;
          a.Rsum = side == 0 ? c->R.dSum : s->gSum;
          a.pairs = s->dPairFb + side;
          a.nPairs = 1;
-         a.nTiles = fb[side].tilesM * fb[side].tilesN;
+         a.pairsOnDevice = 1;
+         a.nTiles = worst.tilesM * worst.tilesN;
          a.thr = survivorThreshold(minScore, minRatio);
          a.top1L = c->L.dTop1;
          a.top2L = c->L.dTop2;
@@ -595,6 +709,7 @@ int v3dm_learned_find(v3dm_ctx *c, v3dm_mlp *m, int topK, float margin, float mi
          a.tileM = scan_variant_tile_m(0);
          a.tileN = scan_variant_tile_n(0);
          a.tileCounter = c->dEpoch + 2;
+         a.persistent = 0;
          a.mapL = side == 0 ? s->flagL : nullptr;
          a.mapR = side == 0 ? nullptr : s->flagR;
          a.inlineInsert = a.nTiles <= scan_variant_resident(0, c->device) ? 1 : 0;
@@ -602,6 +717,8 @@ int v3dm_learned_find(v3dm_ctx *c, v3dm_mlp *m, int topK, float margin, float mi
          if (!a.inlineInsert)
             HIPCHK(c, launch_i8_insert(a, insert_form_for((int64_t)a.nTiles * a.waves, c->device, c->insertForm), st));
       }
+      // the counts for the statistics come back with the match table
+      HIPCHK(c, hipMemcpyAsync(s->hCount, s->dCount, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
    }
    // 6. the exact matcher's ratio test and mutual check on the scored pairs
    FinalizeArgs f{};
@@ -626,6 +743,13 @@ int v3dm_learned_find(v3dm_ctx *c, v3dm_mlp *m, int topK, float margin, float mi
    c->lastNL = nL;
    s->lastNL = nL;
    s->lastK = topK;
+   c->pendingAsync = true;
+   rc = fetch(c, pairs, capacity, count); // waits for the job
+   if (nL > 0 && nR > 0)
+   {
+      nFlag[0] = s->hCount[0];
+      nFlag[1] = s->hCount[1];
+   }
    if (stats)
    {
       stats[0] = nL;
@@ -635,8 +759,7 @@ int v3dm_learned_find(v3dm_ctx *c, v3dm_mlp *m, int topK, float margin, float mi
       stats[4] = nFlag[1];
       stats[5] = nFlag[0] * nR + (int64_t)nL * nFlag[1];
    }
-   c->pendingAsync = true;
-   return fetch(c, pairs, capacity, count);
+   return rc;
 }
 
 int v3dm_learned_candidates(v3dm_ctx *c, int n, int32_t *cand, int *topK)

@@ -50,3 +50,22 @@ def train(net: EmbeddingNet, left: np.ndarray, right: np.ndarray, steps: int, ba
         losses.append(net.train(s, batch, lr * 0.5 * (1 + math.cos(math.pi * frac)), temperature, seed + done))
         done += s
     return np.concatenate(losses)
+
+
+def projection_params(seed: int = 3) -> np.ndarray:
+    """Network parameters that make EmbeddingNet a random orthogonal 128 -> 64
+    projection followed by the L2 normalisation: the same pipeline without
+    what training learns (its ablation).  With P (64 x 128, orthonormal rows),
+    W1 = [P; -P; 0], b1 = 0, W2 = [I, -I, 0], b2 = 0, the hidden layer gives
+    GELU(Px) - GELU(-Px) = Px exactly (GELU(a) - GELU(-a) = a), so the
+    embedding is P x / |P x| up to the BF16 rounding of P."""
+    rng = np.random.default_rng(seed)
+    q, _ = np.linalg.qr(rng.standard_normal((128, 64)))
+    P = q.T.astype(np.float32)  # 64 x 128, orthonormal rows
+    W1 = np.zeros((256, 128), np.float32)
+    W1[:64] = P
+    W1[64:128] = -P
+    W2 = np.zeros((64, 256), np.float32)
+    W2[:, :64] = np.eye(64, dtype=np.float32)
+    W2[:, 64:128] = -np.eye(64, dtype=np.float32)
+    return np.concatenate([W1.ravel(), np.zeros(256, np.float32), W2.ravel(), np.zeros(64, np.float32)])

@@ -323,6 +323,15 @@ class EmbeddingNet:
         assert p.size == self.n_params
         self._check(self._lib.v3dm_mlp_set_params(self._h, ptr(p, ctypes.c_float)))
 
+    def save(self, path: str) -> None:
+        """BF16 weights + FP32 biases with a checked header (v3dm_mlp_save)."""
+        self._check(self._lib.v3dm_mlp_save(self._h, str(path).encode()))
+
+    def load(self, path: str) -> None:
+        """Parameters from a file written by save(); embeddings are then bit
+        for bit those of the saved network."""
+        self._check(self._lib.v3dm_mlp_load(self._h, str(path).encode()))
+
     def loss_grad(self, left: np.ndarray, right: np.ndarray, temperature: float) -> tuple[float, np.ndarray]:
         left = np.ascontiguousarray(left, np.uint8)
         right = np.ascontiguousarray(right, np.uint8)
