@@ -139,6 +139,80 @@ int v3dm_match_db_put(v3dm_match_db *db, int oid, int view0, int view1, const do
 /* Commits and closes (NULL is a no-op). */
 int v3dm_match_db_close(v3dm_match_db *db);
 
+/* --- geometric verification around robust orientation --------------------------- */
+/* After the putative matches the drivers estimate the relative pose of each
+ * pair (computeRobustOrientationMLE: RANSAC on the essential matrix, then
+ * refinePoseFromInliers), form F = K2^-T [T]x R K1^-1, run guided matching
+ * with it, refine again and store the post-guided correspondences with
+ * normalised positions (match_sequence.cpp:229-346,
+ * match_using_voctree.cpp:325-491).  Robust orientation and pose refinement
+ * are outside this library: the integrator supplies them through this hook
+ * (the driver applications load it from a plugin library, --orientation, or
+ * use known poses, --poses).  Matrices are row-major doubles; a pose maps
+ * view-1 camera coordinates X to view-2 camera coordinates R X + T. */
+typedef struct v3dm_orientation_hook
+{
+   void *user;
+   /* computeRobustOrientationMLE(rawCorrs, K1, K2, inlierThreshold, 1000, result,
+    * true, mode) (match_sequence.cpp:229-237): corrs hold pixel positions; writes
+    * R, T and the indices into corrs of the inliers (at most n; the driver keeps
+    * their order) and returns 0, or returns nonzero when no pose was found (the
+    * pair is skipped). */
+   int (*robust_orientation)(void *user, int view1, int view2, const double K1[9], const double K2[9],
+                             double inlierThreshold, int n, const v3dm_correspondence *corrs, double R[9],
+                             double T[3], int32_t *inliers, int *nInliers);
+   /* refinePoseFromInliers(corrs, K1, K2, R, T) (match_sequence.cpp:53-67), called
+    * before and after guided matching (:275, :321): refines R, T in place from pixel
+    * correspondences; nonzero skips the pair.  NULL keeps the pose. */
+   int (*refine_pose)(void *user, int view1, int view2, const double K1[9], const double K2[9], int n,
+                      const v3dm_correspondence *corrs, double R[9], double T[3]);
+} v3dm_orientation_hook;
+
+/* The function a plugin library for the drivers' --orientation option exports
+ * under the name V3DM_ORIENTATION_PLUGIN_SYMBOL: fills the hook, returns 0. */
+typedef int (*v3dm_orientation_plugin_fn)(v3dm_orientation_hook *hook);
+#define V3DM_ORIENTATION_PLUGIN_SYMBOL "v3dm_orientation_plugin"
+
+/* F = invertedMatrix(K2)^T * (crossProductMatrix(T) * R) * invertedMatrix(K1)
+ * (match_sequence.cpp:279-283) with the reference's double operations
+ * (v3d_linear.h:513-530 adjugate inverse, v3d_linearbase.h:1168-1185 products). */
+int v3dm_fundamental_from_pose(const double K1[9], const double K2[9], const double R[9], const double T[3],
+                               double F[9]);
+
+/* Pixel to normalised positions in place: pos = (K^-1 (x, y, 1))[0..1], K1 for
+ * the left measurement, K2 for the right (match_sequence.cpp:327-338,
+ * multiply_A_v_affine). */
+int v3dm_normalize_correspondences(const double K1[9], const double K2[9], int n, v3dm_correspondence *corrs);
+
+/* getEffectiveInlierCount(corrs, width, height) (match_sequence.cpp:34-50):
+ * int(0.5 (coverage(left) + coverage(right)) n + 0.5), coverage as
+ * estimateCoverage (SfM/reconstruction/include/reconstruction_common.h:125-157)
+ * computes it on a 256-wide raster with the squared distance transform of
+ * reconstruction_common2.cpp:17-162; width, height: the left view's image. */
+int v3dm_effective_inlier_count(int n, const v3dm_correspondence *corrs, int width, int height, int *count);
+
+/* The visibility test of match_using_voctree.cpp:397-418: keeps (in order, in
+ * place) the correspondences whose linear triangulation with P0 = K1 [I | 0]
+ * and P1 = K2 [R | T] (v3d_mviewutilities.h:209-258, a least-squares solve)
+ * has positive depth in view 1; *kept = their number. */
+int v3dm_filter_in_front(const double K1[9], const double K2[9], const double R[9], const double T[3], int n,
+                         v3dm_correspondence *corrs, int *kept);
+
+/* CalibrationDatabase(path) (SfM/reconstruction/src/reconstruction_common.cpp:17-38):
+ * the view count, then per view K[0][0] K[0][1] K[0][2] K[1][1] K[1][2], four
+ * distortion terms, width and height.  Fills K (9 per view, the rest of K
+ * the identity's), distortion (4 per view) and dims (width, height per view);
+ * each may be NULL.  V3DM_ERR_CAPACITY with *nViews set when cap is smaller. */
+int v3dm_calibration_read(const char *path, int cap, double *K, double *distortion, int32_t *dims, int *nViews);
+
+/* The inlier rule of the drivers' known-pose orientation (--poses): indices of
+ * the correspondences whose squared Sampson distance under F, in double,
+ * is at most threshold^2 -- the epipolar test guided matching applies
+ * (v3d_cudamatching.cu:266-290).  No counterpart in the reference, whose
+ * inliers come from its RANSAC. */
+int v3dm_sampson_inliers(const double F[9], double threshold, int n, const v3dm_correspondence *corrs,
+                         int32_t *inliers, int *nInliers);
+
 #ifdef __cplusplus
 }
 #endif

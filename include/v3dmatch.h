@@ -211,8 +211,31 @@ int v3dm_batch_set_images_u8(v3dm_ctx *ctx, int nImages, const int32_t *counts, 
  * rescoring, whose per-pair tables equal the plain FP32 rule's. */
 int v3dm_batch_set_images_f32(v3dm_ctx *ctx, int nImages, const int32_t *counts, const float *desc);
 
-/* Sets the pairs to match: pairImages[2p] = left image, [2p+1] = right image. */
+/* Sets the pairs to match: pairImages[2p] = left image, [2p+1] = right image.
+ * Clears the per-pair guided masks. */
 int v3dm_batch_set_pairs(v3dm_ctx *ctx, int nPairs, const int32_t *pairImages);
+
+/* Feature positions of the uploaded images: interleaved (x, y) floats per
+ * feature, images in the order and with the counts of the last
+ * v3dm_batch_set_images_* call -- the leftPositions / rightPositions that
+ * runGuidedMatching takes (v3d_cudamatching.h:48-52; the drivers pass each
+ * view's SIFT_Feature::position, match_sequence.cpp:285-292).  Needed by
+ * v3dm_batch_set_guides; cleared by v3dm_batch_set_images_*. */
+int v3dm_batch_set_positions(v3dm_ctx *ctx, const float *positions);
+
+/* Guided matching for every pair of the current pair list in one job: for
+ * pair p, kind[p] = 0 leaves the pair's findPutativeMatches list, 1 gives
+ * runGuidedMatching(F, distanceThreshold[p], ...) with the row-major
+ * fundamental matrix F = M[9p .. 9p+8] (v3d_cudamatching.cpp:129-191,
+ * v3d_cudamatching.cu:259-293), 2 gives runGuidedHomographyMatching with the
+ * homography H = M[9p .. 9p+8] (.cpp:193-255, .cu:295-325).  Each pair's
+ * table then equals findPutativeMatches followed by that one guided call on
+ * the pair -- what the drivers do after robust orientation
+ * (match_sequence.cpp:275-293).  The masks stay for later jobs until the next
+ * v3dm_batch_set_pairs / set_images or a call with kind = NULL (clears them).
+ * Requires v3dm_batch_set_positions.  No counterpart in the reference, which
+ * runs one guided call per pair. */
+int v3dm_batch_set_guides(v3dm_ctx *ctx, const int32_t *kind, const double *M, const float *distanceThreshold);
 
 /* Runs the job.  matchOfLeft receives, for pair p, entries
  * [offsets[p], offsets[p+1]) = per-left-feature right index or -1 (the

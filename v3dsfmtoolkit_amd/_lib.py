@@ -69,6 +69,8 @@ _SIGNATURES = {
     "v3dm_batch_set_images_u8": (ctypes.c_int, [_ctx_p, ctypes.c_int, _i32_p, _u8_p]),
     "v3dm_batch_set_images_f32": (ctypes.c_int, [_ctx_p, ctypes.c_int, _i32_p, _f32_p]),
     "v3dm_batch_set_pairs": (ctypes.c_int, [_ctx_p, ctypes.c_int, _i32_p]),
+    "v3dm_batch_set_positions": (ctypes.c_int, [_ctx_p, _f32_p]),
+    "v3dm_batch_set_guides": (ctypes.c_int, [_ctx_p, _i32_p, _f64_p, _f32_p]),
     "v3dm_batch_match": (ctypes.c_int, [_ctx_p, ctypes.c_float, ctypes.c_float, _i32_p, _i64_p]),
     "v3dm_batch_match_async": (ctypes.c_int, [_ctx_p, ctypes.c_float, ctypes.c_float]),
     "v3dm_batch_fetch": (ctypes.c_int, [_ctx_p, _i32_p, _i64_p]),
@@ -130,6 +132,15 @@ _SIGNATURES = {
     "v3dm_match_db_put": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f64_p, _f64_p,
                                          ctypes.c_int, ctypes.c_void_p]),
     "v3dm_match_db_close": (ctypes.c_int, [ctypes.c_void_p]),
+    "v3dm_fundamental_from_pose": (ctypes.c_int, [_f64_p, _f64_p, _f64_p, _f64_p, _f64_p]),
+    "v3dm_normalize_correspondences": (ctypes.c_int, [_f64_p, _f64_p, ctypes.c_int, ctypes.c_void_p]),
+    "v3dm_effective_inlier_count": (ctypes.c_int, [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                                   _c_int_p]),
+    "v3dm_filter_in_front": (ctypes.c_int, [_f64_p, _f64_p, _f64_p, _f64_p, ctypes.c_int, ctypes.c_void_p,
+                                            _c_int_p]),
+    "v3dm_calibration_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, _f64_p, _f64_p, _i32_p, _c_int_p]),
+    "v3dm_sampson_inliers": (ctypes.c_int, [_f64_p, ctypes.c_double, ctypes.c_int, ctypes.c_void_p, _i32_p,
+                                            _c_int_p]),
     # include/v3dm_voctree.h
     "v3dm_voctree_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32_p,
                                            ctypes.POINTER(ctypes.c_void_p)]),

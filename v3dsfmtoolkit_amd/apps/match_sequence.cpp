@@ -18,20 +18,31 @@
 // (u/512) go to the exact int8 pool, others to the BF16 + FP32-rescoring pool;
 // each pair's list equals findPutativeMatches on that pair.
 //
-// Output: the raw correspondences the reference hands to its robust
-// orientation step (:205-227, computeRobustOrientationMLE) for every pair with
-// at least MINIMUM_RAW_CORRESPONDENCES of them (:229), written as PairwiseMatch
-// rows (rotation identity, translation zero: that step fills them) with
-// pixel positions into the tables matches_data / matches_list of an SQLite file
-// (default putative_matches.db), OIDs 0, 1, ... in pair order
-// (:340-346).  The per-pair log lines follow the reference's.
+// Output: without an orientation source, the raw correspondences the
+// reference hands to its robust orientation step (:205-227,
+// computeRobustOrientationMLE) for every pair with at least
+// MINIMUM_RAW_CORRESPONDENCES of them (:229), written as PairwiseMatch rows
+// (rotation identity, translation zero) with pixel positions to
+// putative_matches.db.  With one (--orientation LIB: a plugin exporting
+// v3dm_orientation_plugin; --poses FILE: known relative poses), the
+// reference's geometric stage follows (:229-346): robust orientation, the
+// inlier checks (MINIMUM_INLIER_CORRESPONDENCES, MINIMUM_INLIER_MATCHES_RATIO),
+// guided matching with F = K2^-T [T]x R K1^-1 (USE_GUIDED_MATCHING; all
+// accepted pairs in one batched job), the effective-inlier check
+// (USE_EFFECTIVE_INLIERS), and rows with the pose and normalised positions
+// in pairwise_matches.db, the reference's file.  Intrinsics and image sizes
+// come from calibration_db.txt (--calibration).  Tables matches_data /
+// matches_list, OIDs 0, 1, ... in pair order (:340-346).  The per-pair log
+// lines follow the reference's.
 //
-// Options: --db PATH (default putative_matches.db), --no-db, --device N,
-//          --pairs-per-job N (default 4096), --autotune (time the scan tile
-//          variants on the first job instead of 256 x 1024 tiles), --quiet.
+// Options: --db PATH, --no-db, --orientation LIB, --poses FILE,
+//          --calibration PATH, --device N, --pairs-per-job N (default 4096),
+//          --autotune (time the scan tile variants on the first job instead
+//          of 256 x 1024 tiles), --quiet.
 #include <cstdlib>
 #include <fstream>
 #include <iostream>
+#include <memory>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -46,7 +57,8 @@ void usage(const char *argv0)
 {
    std::cerr << "Usage: " << argv0 << " [options] <image list file> <file.conf> <radius>\n"
              << "       " << argv0 << " [options] --pairs <pairs file> <image list file> <file.conf>\n"
-             << "options: --db PATH | --no-db | --device N | --pairs-per-job N | --autotune | --quiet" << std::endl;
+             << "options: --db PATH | --no-db | --orientation LIB | --poses FILE | --calibration PATH |\n"
+             << "         --device N | --pairs-per-job N | --autotune | --quiet" << std::endl;
 }
 
 // "a b" or "a -- b;" per line; other lines ("graph {", "}") are skipped.
@@ -76,7 +88,8 @@ std::vector<int32_t> readPairs(const char *path, int nViews)
 int main(int argc, char **argv)
 {
    MatchOptions o;
-   o.dbPath = "putative_matches.db";
+   std::string orientationLib, posesFile;
+   bool dbGiven = false;
    std::string pairsFile;
    std::vector<const char *> pos;
    for (int i = 1; i < argc; ++i)
@@ -90,8 +103,11 @@ int main(int argc, char **argv)
          }
          return argv[++i];
       };
-      if (a == "--db") o.dbPath = value();
-      else if (a == "--no-db") o.dbPath.clear();
+      if (a == "--db") o.dbPath = value(), dbGiven = true;
+      else if (a == "--no-db") o.dbPath.clear(), dbGiven = true;
+      else if (a == "--orientation") orientationLib = value();
+      else if (a == "--poses") posesFile = value();
+      else if (a == "--calibration") o.geo.calibPath = value();
       else if (a == "--device") o.device = std::atoi(value());
       else if (a == "--pairs-per-job") o.pairsPerJob = std::atoi(value());
       else if (a == "--pairs") pairsFile = value();
@@ -113,6 +129,19 @@ int main(int argc, char **argv)
    const Config cf(pos[1]);
    o.distRatio = cf.get("MATCHING_SCORE_RATIO", 0.8f);
    o.nMinRawCorrs = cf.get("MINIMUM_RAW_CORRESPONDENCES", 100);
+   // the geometric stage: robust orientation from a plugin or from known poses
+   std::unique_ptr<KnownPoses> known;
+   if (!orientationLib.empty() && !posesFile.empty()) die("--orientation and --poses exclude each other");
+   if (!orientationLib.empty()) loadOrientationPlugin(orientationLib, o.geo.hook);
+   if (!posesFile.empty())
+   {
+      known.reset(new KnownPoses(posesFile));
+      o.geo.hook = known->hook();
+   }
+   o.geo.enabled = o.geo.hook.robust_orientation != nullptr;
+   o.geo.visibilityTest = false;
+   readGeometryConfig(cf, o.nMinRawCorrs, true, o.geo);
+   if (!dbGiven) o.dbPath = o.geo.enabled ? "pairwise_matches.db" : "putative_matches.db";
    const int nMaxFeatures = cf.get("MATCHING_MAX_FEATURES", 7168);
 
    std::vector<View> views = readImageList(pos[0]);

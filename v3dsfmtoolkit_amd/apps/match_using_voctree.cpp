@@ -16,16 +16,23 @@
 // Stage 2 (:251-323): the candidate pairs are matched as batched jobs with
 // the reference's rule (minScore 0.9, MATCHING_SCORE_RATIO) and the pairs
 // with at least MINIMUM_RAW_CORRESPONDENCES raw correspondences are written
-// to the match database as in v3dm_match_sequence.
+// to the match database as in v3dm_match_sequence.  With an orientation
+// source (--orientation LIB, --poses FILE) the geometric stage of :325-491
+// follows as in v3dm_match_sequence, plus the visibility test of :397-418
+// (USE_EFFECTIVE_INLIERS defaults to false here, :122); its panorama and
+// homography databases (:493-560) are not produced.
 //
-// Options: --db PATH (default putative_matches.db), --no-db, --dot PATH,
-//          --tree-shape DEPTH BRANCH (default 3 50), --device N,
-//          --pairs-per-job N (default 4096), --keys-per-job N (default 4194304),
-//          --autotune (time the scan tile variants on the first job), --quiet.
+// Options: --db PATH (default putative_matches.db, pairwise_matches.db with
+//          an orientation source), --no-db, --dot PATH, --orientation LIB,
+//          --poses FILE, --calibration PATH, --tree-shape DEPTH BRANCH
+//          (default 3 50), --device N, --pairs-per-job N (default 4096),
+//          --keys-per-job N (default 4194304), --autotune (time the scan tile
+//          variants on the first job), --quiet.
 #include <algorithm>
 #include <cstdlib>
 #include <fstream>
 #include <iostream>
+#include <memory>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -40,7 +47,8 @@ namespace {
 void usage(const char *argv0)
 {
    std::cerr << "Usage: " << argv0 << " [options] <vocabulary tree file> <image list file> <file.conf>\n"
-             << "options: --db PATH | --no-db | --dot PATH | --tree-shape DEPTH BRANCH | --device N |\n"
+             << "options: --db PATH | --no-db | --dot PATH | --orientation LIB | --poses FILE | --calibration PATH |\n"
+             << "         --tree-shape DEPTH BRANCH | --device N |\n"
              << "         --pairs-per-job N | --keys-per-job N | --autotune | --quiet" << std::endl;
 }
 
@@ -49,7 +57,8 @@ void usage(const char *argv0)
 int main(int argc, char **argv)
 {
    MatchOptions o;
-   o.dbPath = "putative_matches.db";
+   std::string orientationLib, posesFile;
+   bool dbGiven = false;
    std::string dotPath = "putative_EG.dot";
    int depth = 3, branch = 50;
    int64_t keysPerJob = 4194304;
@@ -65,8 +74,11 @@ int main(int argc, char **argv)
          }
          return argv[++i];
       };
-      if (a == "--db") o.dbPath = value();
-      else if (a == "--no-db") o.dbPath.clear();
+      if (a == "--db") o.dbPath = value(), dbGiven = true;
+      else if (a == "--no-db") o.dbPath.clear(), dbGiven = true;
+      else if (a == "--orientation") orientationLib = value();
+      else if (a == "--poses") posesFile = value();
+      else if (a == "--calibration") o.geo.calibPath = value();
       else if (a == "--dot") dotPath = value();
       else if (a == "--tree-shape")
       {
@@ -94,6 +106,19 @@ int main(int argc, char **argv)
    const Config cf(pos[2]);
    o.distRatio = cf.get("MATCHING_SCORE_RATIO", 0.8f);
    o.nMinRawCorrs = cf.get("MINIMUM_RAW_CORRESPONDENCES", 100);
+   // the geometric stage: robust orientation from a plugin or from known poses
+   std::unique_ptr<KnownPoses> known;
+   if (!orientationLib.empty() && !posesFile.empty()) die("--orientation and --poses exclude each other");
+   if (!orientationLib.empty()) loadOrientationPlugin(orientationLib, o.geo.hook);
+   if (!posesFile.empty())
+   {
+      known.reset(new KnownPoses(posesFile));
+      o.geo.hook = known->hook();
+   }
+   o.geo.enabled = o.geo.hook.robust_orientation != nullptr;
+   o.geo.visibilityTest = true;
+   readGeometryConfig(cf, o.nMinRawCorrs, false, o.geo);
+   if (!dbGiven) o.dbPath = o.geo.enabled ? "pairwise_matches.db" : "putative_matches.db";
    const int nMaxFeatures = cf.get("MATCHING_MAX_FEATURES", 7168);
    const int nBestDocuments = cf.get("POTENTIAL_VIEWS_COUNT", 20);
    const float pMatch = (float)cf.get("VISUAL_WORD_MATCH_PROPABILITY", 0.1);

@@ -288,6 +288,11 @@ __device__ inline float dot_seq(const float *__restrict__ x, const float *__rest
 __device__ inline void rescore_insert(const ScanArgsF &a, const PairDesc &pd, uint32_t epoch, int il, int jr)
 {
    if (il >= pd.nL || jr >= pd.nR) return;
+   if (pd.guide > 0)
+   {
+      const Mask &mk = a.masks[pd.guide - 1];
+      if (mask_hit(mk, mk.lpos[2 * il], mk.lpos[2 * il + 1], mk.rpos[2 * jr], mk.rpos[2 * jr + 1])) return;
+   }
    for (int m = 0; m < a.nMasks; ++m)
    {
       const Mask &mk = a.masks[m];

@@ -58,6 +58,11 @@ __device__ __forceinline__ void survivor(const ScanArgs &a, const PairDesc &pd, 
    // rows of a gathered pool stand for the features mapL / mapR name
    if (a.mapL) il = a.mapL[il];
    if (a.mapR) jr = a.mapR[jr];
+   if (pd.guide > 0)
+   {
+      const Mask &mk = a.masks[pd.guide - 1];
+      if (mask_hit(mk, mk.lpos[2 * il], mk.lpos[2 * il + 1], mk.rpos[2 * jr], mk.rpos[2 * jr + 1])) return;
+   }
    for (int m = 0; m < a.nMasks; ++m)
    {
       const Mask &mk = a.masks[m];

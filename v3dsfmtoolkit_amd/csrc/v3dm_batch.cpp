@@ -23,6 +23,7 @@ struct BatchState
 {
    // images
    int nImages = 0;
+   std::vector<int> given;       // feature counts as given (rows of each image in the caller's arrays)
    std::vector<int> counts;      // feature counts after rounding to multiples of 32
    std::vector<int64_t> imgOff;  // first pool row of each image
    int64_t poolRows = 0;
@@ -72,6 +73,15 @@ struct BatchState
    uint32_t hostEpoch = 1;
    bool keysFresh = false;
 
+   // guided matching per pair: positions at the pool's row offsets (2 floats per row)
+   // and one mask per user pair (PairDesc::guide = pair + 1 when the pair has one)
+   float *dPos = nullptr;
+   int64_t posCap = 0;
+   bool hasPos = false;
+   std::vector<int32_t> guideKind; // per user pair; empty: no guided masks
+   Mask *dGuides = nullptr;
+   int64_t guidesCap = 0;
+
    bool useGraph = false;
    hipGraphExec_t exec = nullptr;
    bool graphValid = false;
@@ -99,6 +109,8 @@ void freeBatch(v3dm_ctx *c)
    dfree(b->dSurv);
    dfree(b->dSurvCount);
    dfree(b->dCounters);
+   dfree(b->dPos);
+   dfree(b->dGuides);
    if (b->hMatch) (void)hipHostFree(b->hMatch);
    delete b;
    c->batch = nullptr;
@@ -147,6 +159,7 @@ void buildTable(BatchState *b, int v)
       if (d.nL == 0 || d.nR == 0) d.tilesM = d.tilesN = 0;
       d.tileBase = tiles;
       d.finBase = fin;
+      d.guide = (!b->guideKind.empty() && b->guideKind[p] != 0) ? p + 1 : 0;
       tiles += d.tilesM * d.tilesN;
       fin += std::max(1, (d.nL + 255) / 256);
       b->tilePair.insert(b->tilePair.end(), (size_t)d.tilesM * d.tilesN, i);
@@ -217,7 +230,7 @@ void fillArgs(v3dm_ctx *c, BatchState *b, float minScore, float minRatio, ScanAr
    a.top1R = k + 2 * b->keyCapL;
    a.top2R = k + 2 * b->keyCapL + b->keyCapR;
    a.epoch = b->dCounters;
-   a.masks = nullptr;
+   a.masks = b->guideKind.empty() ? nullptr : b->dGuides; // indexed by PairDesc::guide
    a.nMasks = 0;
    a.surv = b->dSurv;
    a.survCount = b->dSurvCount;
@@ -269,6 +282,8 @@ void fillArgsF(v3dm_ctx *c, BatchState *b, float minScore, float minRatio, ScanA
    a.top1R = k + 2 * b->keyCapL;
    a.top2R = k + 2 * b->keyCapL + b->keyCapR;
    a.epoch = b->dCounters;
+   a.masks = b->guideKind.empty() ? nullptr : b->dGuides; // indexed by PairDesc::guide
+   a.nMasks = 0;
    a.surv = b->dSurv;
    a.survCount = b->dSurvCount;
    a.survCap = c->survCap;
@@ -551,6 +566,9 @@ int v3dm_batch_set_images_u8(v3dm_ctx *c, int nImages, const int32_t *counts, co
    if (rc) return rc;
    b->isFloat = false;
    b->nImages = nImages;
+   b->given.assign(counts, counts + nImages);
+   b->hasPos = false;
+   b->guideKind.clear();
    b->counts = cnt;
    b->imgOff = off;
    b->poolRows = rows;
@@ -622,6 +640,9 @@ int v3dm_batch_set_images_f32(v3dm_ctx *c, int nImages, const int32_t *counts, c
    memcpy(&b->norm2Max, &bits, sizeof(bits));
    b->isFloat = true;
    b->nImages = nImages;
+   b->given.assign(counts, counts + nImages);
+   b->hasPos = false;
+   b->guideKind.clear();
    b->counts = cnt;
    b->imgOff = off;
    b->poolRows = rows;
@@ -649,6 +670,7 @@ int v3dm_batch_set_pairs(v3dm_ctx *c, int nPairs, const int32_t *pairImages)
          return fail(c, V3DM_ERR_ARG, "batch_set_pairs: image index %d out of range", pairImages[p]);
    b->nPairs = nPairs;
    b->pairImg.assign(pairImages, pairImages + 2 * nPairs);
+   b->guideKind.clear();
    b->outOff.assign(nPairs + 1, 0);
    b->rOff.assign(nPairs + 1, 0);
    for (int p = 0; p < nPairs; ++p)
@@ -686,6 +708,102 @@ int v3dm_batch_set_pairs(v3dm_ctx *c, int nPairs, const int32_t *pairImages)
       buildTable(b, b->wantVariant);
       return uploadTable(c, b);
    }
+   return syncStream(c);
+}
+
+int v3dm_batch_set_positions(v3dm_ctx *c, const float *positions)
+{
+   if (!c) return V3DM_ERR_ARG;
+   BatchState *b = c->batch;
+   if (!b || b->nImages == 0) return fail(c, V3DM_ERR_STATE, "batch_set_positions: no images uploaded");
+   int64_t total = 0;
+   for (int g : b->given) total += g;
+   if (total > 0 && !positions) return fail(c, V3DM_ERR_ARG, "batch_set_positions: position pointer is NULL");
+   HIPCHK(c, hipSetDevice(c->device));
+   if (b->pending)
+   {
+      int rc = syncStream(c);
+      if (rc) return rc;
+      b->pending = false;
+   }
+   // the pool's layout: image i's first counts[i] positions at rows imgOff[i] (padding rows unused)
+   std::vector<float> staged(2 * (size_t)std::max<int64_t>(b->poolRows, 1), 0.0f);
+   int64_t src = 0;
+   for (int i = 0; i < b->nImages; ++i)
+   {
+      std::copy(positions + 2 * src, positions + 2 * (src + b->counts[i]), staged.begin() + 2 * b->imgOff[i]);
+      src += b->given[i];
+   }
+   int rc = growDevice(c, b->dPos, b->posCap, 2 * std::max<int64_t>(b->poolRows, 1));
+   if (rc) return rc;
+   HIPCHK(c, hipMemcpyAsync(b->dPos, staged.data(), sizeof(float) * staged.size(), hipMemcpyHostToDevice, c->stream));
+   rc = syncStream(c);
+   if (rc) return rc;
+   b->hasPos = true;
+   return V3DM_OK;
+}
+
+int v3dm_batch_set_guides(v3dm_ctx *c, const int32_t *kind, const double *M, const float *distanceThreshold)
+{
+   if (!c) return V3DM_ERR_ARG;
+   BatchState *b = c->batch;
+   if (!b || b->nImages == 0) return fail(c, V3DM_ERR_STATE, "batch_set_guides: no images uploaded");
+   HIPCHK(c, hipSetDevice(c->device));
+   if (b->pending)
+   {
+      int rc = syncStream(c);
+      if (rc) return rc;
+      b->pending = false;
+   }
+   std::vector<int32_t> kinds;
+   std::vector<Mask> masks;
+   if (kind && b->nPairs > 0)
+   {
+      kinds.assign(kind, kind + b->nPairs);
+      bool any = false;
+      for (int p = 0; p < b->nPairs; ++p)
+      {
+         if (kinds[p] < 0 || kinds[p] > 2)
+            return fail(c, V3DM_ERR_ARG, "batch_set_guides: pair %d has mask kind %d (0, 1 or 2)", p, kinds[p]);
+         any = any || kinds[p] != 0;
+      }
+      if (!any) kinds.clear();
+   }
+   if (!kinds.empty())
+   {
+      if (!M || !distanceThreshold) return fail(c, V3DM_ERR_ARG, "batch_set_guides: matrix or threshold pointer is NULL");
+      if (!b->hasPos) return fail(c, V3DM_ERR_STATE, "batch_set_guides: no feature positions (v3dm_batch_set_positions)");
+      masks.assign(b->nPairs, Mask{});
+      for (int p = 0; p < b->nPairs; ++p)
+      {
+         if (!kinds[p]) continue;
+         Mask &mk = masks[p];
+         mk.kind = kinds[p];
+         const float t = distanceThreshold[p];
+         mk.sqrThr = t * t;                                           // v3d_cudamatching.cu:414 (float)
+         for (int i = 0; i < 9; ++i) mk.m[i] = (float)M[9 * (size_t)p + i]; // v3d_cudamatching.cpp:161-164
+         mk.lpos = b->dPos + 2 * b->imgOff[b->pairImg[2 * p]];
+         mk.rpos = b->dPos + 2 * b->imgOff[b->pairImg[2 * p + 1]];
+      }
+      int rc = growDevice(c, b->dGuides, b->guidesCap, (int64_t)b->nPairs);
+      if (rc) return rc;
+      HIPCHK(c, hipMemcpyAsync(b->dGuides, masks.data(), sizeof(Mask) * masks.size(), hipMemcpyHostToDevice,
+                               c->stream));
+   }
+   b->guideKind.swap(kinds);
+   if (b->variant >= 0)
+   {
+      // the tile table is built: refresh its guide indices
+      for (int i = 0; i < b->nPairs; ++i)
+      {
+         const int p = b->order[i];
+         b->table[i].guide = (!b->guideKind.empty() && b->guideKind[p] != 0) ? p + 1 : 0;
+      }
+      if (b->nPairs > 0)
+         HIPCHK(c, hipMemcpyAsync(b->dPairs, b->table.data(), sizeof(PairDesc) * b->nPairs, hipMemcpyHostToDevice,
+                                  c->stream));
+   }
+   b->graphValid = false;
    return syncStream(c);
 }
 

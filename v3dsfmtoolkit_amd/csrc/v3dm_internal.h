@@ -28,6 +28,7 @@ struct PairDesc
    int tilesM, tilesN;   // scan tiles along left / right
    int tileBase;         // index of this pair's first scan tile
    int finBase;          // index of this pair's first finalize block
+   int guide;            // 0: no per-pair mask; k > 0: the batch's guided mask masks[k - 1]
 };
 
 // Guided-matching mask, applied to candidate (left p, right q) positions.

@@ -201,6 +201,28 @@ class SIFTMatching:
         check(self._lib.v3dm_batch_set_pairs(self._ctx, len(pairs), ptr(pairs, ctypes.c_int32)), self._ctx)
         self._pairs = pairs
 
+    def setPositions(self, positions) -> None:
+        """Feature positions of the uploaded images (list of (n_i, 2) float arrays, the
+        counts of setImages): what runGuidedMatching takes as left/right positions."""
+        pos = np.ascontiguousarray(np.concatenate([np.asarray(x, np.float32).reshape(-1, 2) for x in positions])
+                                   if len(positions) else np.zeros((0, 2), np.float32))
+        check(self._lib.v3dm_batch_set_positions(self._ctx, ptr(pos, ctypes.c_float)), self._ctx)
+
+    def setGuides(self, kinds=None, matrices=None, thresholds=None) -> None:
+        """Per-pair guided masks of the current pair list: kind 0 none, 1 epipolar (F),
+        2 homography (H); matrices (nPairs, 3, 3) float64, thresholds (nPairs,) float32.
+        kinds=None clears them."""
+        if kinds is None:
+            check(self._lib.v3dm_batch_set_guides(self._ctx, None, None, None), self._ctx)
+            return
+        k = np.ascontiguousarray(np.asarray(kinds, np.int32).reshape(-1))
+        m = np.ascontiguousarray(np.asarray(matrices, np.float64).reshape(-1, 9))
+        t = np.ascontiguousarray(np.asarray(thresholds, np.float32).reshape(-1))
+        if not (len(k) == len(m) == len(t) == len(self._pairs)):
+            raise ValueError("setGuides: one kind, matrix and threshold per pair")
+        check(self._lib.v3dm_batch_set_guides(self._ctx, ptr(k, ctypes.c_int32), ptr(m, ctypes.c_double),
+                                              ptr(t, ctypes.c_float)), self._ctx)
+
     def batchInfo(self) -> dict:
         n, nl, nt, ps = ctypes.c_int(), ctypes.c_int64(), ctypes.c_int(), ctypes.c_int64()
         check(self._lib.v3dm_batch_info(self._ctx, ctypes.byref(n), ctypes.byref(nl), ctypes.byref(nt),
