@@ -84,13 +84,16 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run a plain device copy of this size (scales memory counters)")
-    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5", "voctree", "sequence", "voctree-driver"],
+    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5", "voctree", "sequence", "voctree-driver",
+                                           "overflow"],
                     default="c2",
                     help="c2: single pair 8192^2 (default line); c3: 256 uneven pairs in one batch (+ float "
                          "variant); c4: learned path on 64 pairs of 16384^2; c5: single pair 65536^2; voctree: "
                          "vocabulary-tree quantisation of 64 views x 7168 features (depth 3, branch factor 50); "
                          "sequence: the match_sequence counterpart on 64 feature files, radius 3; voctree-driver: "
-                         "the match_using_voctree counterpart on the same files")
+                         "the match_using_voctree counterpart on the same files; overflow: the survivor-overflow "
+                         "cases (same-order near-duplicate frames, identical sets, ratio test off) against the "
+                         "benchmark pair")
     ap.add_argument("--pairs", type=int, default=0, help="number of pairs for c3 / c4 (0: the workload's own)")
     ap.add_argument("--scan-variant", type=int, default=-1,
                     help="single-pair int8 scan variant (-1: the library measures the fastest)")
@@ -109,7 +112,8 @@ def main():
 
     if args.workload != "c2":
         out = {"c3": run_c3, "c4": run_c4, "c5": run_c5, "voctree": run_voctree,
-               "sequence": run_sequence, "voctree-driver": run_voctree_driver}[args.workload](args, v)
+               "sequence": run_sequence, "voctree-driver": run_voctree_driver,
+               "overflow": run_overflow}[args.workload](args, v)
         print(json.dumps(out), flush=True)
         return
     run_c2(args, v)
@@ -618,6 +622,130 @@ def run_c5(args, v):
     out["cpu_baseline"] = {"value": cpu, "unit": "descriptor pairs scored/s", "cores": threads, "kind": "port",
                            "sample": "2048 left rows against all 65536 rights, oracle blocked rule"}
     return out
+
+
+def _survivor_threshold(min_score, min_ratio):
+    """The scan's integer survivor threshold (scores in units of 2^-18): a score at or
+    below it cannot change a decision; -1 keeps every score (a tiny positive minRatio),
+    2^28 none (a NaN threshold accepts nothing)."""
+    ms, mr = float(np.float32(min_score)), float(np.float32(min_ratio))
+    if mr < 0:
+        t = ms
+    elif mr > 0:
+        t = min(ms, 1.0 - (1.0 - ms) / (mr * (1.0 - 1e-5)))
+    elif mr == 0:
+        t = max(ms, 1.0)  # only bests above 1 give a negative ratio
+    else:
+        return 1 << 28  # NaN: nothing is accepted
+    if t != t:
+        return 1 << 28
+    return min(max(-1, int(np.floor(t * 262144.0))), 1 << 28)
+
+
+def _region_stats(L, R, thr, waves, tile_n):
+    """Survivors (exact integer scores > thr, from float32 products of the uint8 rows,
+    exact below 2^24) and how many of the scan's (tile, wave) regions -- 64 left rows x
+    tile_n right rows, capacity 64 per 512 columns -- overflow."""
+    cap = 64 * max(1, tile_n // 512)
+    Rt = R.astype(np.float32).T
+    surv, over, regions = 0, 0, 0
+    for r0 in range(0, len(L), 2048):
+        S = L[r0:r0 + 2048].astype(np.float32) @ Rt
+        hit = S > thr
+        surv += int(hit.sum())
+        nb = hit.shape[0] // 64
+        for c0 in range(0, hit.shape[1], tile_n):
+            cnt = hit[:nb * 64, c0:c0 + tile_n].reshape(nb, 64, -1).sum(axis=(1, 2))
+            over += int((cnt > cap).sum())
+            regions += nb
+    return surv, over, regions, cap
+
+
+def run_overflow(args, v):
+    """The survivor-overflow cliff: pairs whose scores pass the survivor filter far more
+    often than the benchmark pair's, against that pair (8192 x 8192, 40% noisy copies).
+    A (tile, wave) region holds 64 survivors per 512 right rows; a region that overflows
+    is recomputed exactly from the descriptors (v_dot4 over its 64 x tile-width block),
+    by the scan wave itself when the grid fits the device, else by k_i8_insert.
+    Cases: consecutive frames whose features come in the same order (every right row a
+    noisy copy of the left row with its index: one survivor per row, all on the diagonal
+    of the tile grid); identical sets (L = R, same order); minRatio 0 (only bests above 1
+    can pass, so the filter is max(minScore, 1)); a tiny ratio (minScore 0.9, minRatio
+    0.05: no filter is provable, every score survives -- the int8 scan forced, beside the
+    plain path the automatic choice takes); and the same-order copies at 16384 x 16384.
+    Each case also reports the plain path's time and the automatic kernel choice."""
+    def case(name, L, R, min_score, min_ratio, steps):
+        m = v.SIFTMatching(0)
+        m.enableRoundingWarning(False)
+        m.allocate(len(L), len(R))
+        m.setLeftFeatures(len(L), L)
+        m.setRightFeatures(len(R), R)
+        m.setKernel(v.KERNEL_I8)
+        got = m.findPutativeMatches(min_score, min_ratio)
+        for _ in range(3):
+            m.findAsync(min_score, min_ratio)
+        m.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            m.findAsync(min_score, min_ratio)
+        m.synchronize()
+        wall = (time.perf_counter() - t0) / steps
+        m.profileEnable(True)  # kernel times: the same jobs again, with events bound to the scan
+        for _ in range(steps):
+            m.findAsync(min_score, min_ratio)
+        m.synchronize()
+        scan_ms, sel_ms, jobs = m.profileRead()
+        m.profileEnable(False)
+        variant, inline = m.lastScanVariant(), m.lastScanInline() == 1
+        w, _, tn = v.scan_variants()[variant]
+        m.setKernel(v.KERNEL_PLAIN)
+        same = m.findPutativeMatches(min_score, min_ratio) == got
+        t0 = time.perf_counter()
+        for _ in range(3):
+            m.findAsync(min_score, min_ratio)
+        m.synchronize()
+        plain = (time.perf_counter() - t0) / 3
+        m.setKernel(v.KERNEL_AUTO)
+        auto_same = m.findPutativeMatches(min_score, min_ratio) == got
+        auto_kernel = {v.KERNEL_I8: "int8", v.KERNEL_PLAIN: "plain", v.KERNEL_BF16: "bf16"}[m.lastKernel()]
+        m.close()
+        surv, over, regions, cap = _region_stats(L, R, _survivor_threshold(min_score, min_ratio), w, tn)
+        progress(f"overflow case {name}: {wall * 1e3:.4f} ms/job, {over} of {regions} regions overflow")
+        return {"n": [len(L), len(R)], "rule": [min_score, min_ratio], "ms_per_job": wall * 1e3,
+                "scan_us": scan_ms / max(jobs, 1) * 1e3, "select_us": sel_ms / max(jobs, 1) * 1e3,
+                "scan_kernel": _scan_name(v, variant, inline),
+                "insertion": "in the scan" if inline else "k_i8_insert", "survivors": surv,
+                "survivors_per_left_row": surv / len(L), "regions": regions, "regions_overflowed": over,
+                "region_capacity": cap, "matches": len(got), "equals_plain_kernel": same,
+                "plain_ms_per_job": plain * 1e3, "automatic_choice": auto_kernel,
+                "automatic_choice_equal": auto_same}
+
+    n = 8192
+    base = v.synth_pair(v.synth_seed(1, 0), n, n, 0.4, 0.05)
+    dup = v.synth_pair(v.synth_seed(7, 0), n, n, 1.0, 0.05)
+    ordered = np.empty_like(dup["R"])
+    ordered[dup["src"]] = dup["R"]  # right row i: the noisy copy of left row i
+    res = {"c2": case("c2", base["L"], base["R"], 0.8, 0.8, args.steps),
+           "same_order_copies": case("same_order_copies", dup["L"], ordered, 0.8, 0.8, args.steps),
+           "identical": case("identical", base["L"], base["L"], 0.8, 0.8, args.steps),
+           "min_ratio_0": case("min_ratio_0", base["L"], base["R"], 0.8, 0.0, args.steps),
+           "tiny_ratio_no_filter": case("tiny_ratio_no_filter", base["L"], base["R"], 0.9, 0.05,
+                                        max(args.steps // 20, 3))}
+    big = v.synth_pair(v.synth_seed(7, 1), 16384, 16384, 1.0, 0.05)
+    ordered = np.empty_like(big["R"])
+    ordered[big["src"]] = big["R"]
+    res["same_order_copies_16384"] = case("same_order_copies_16384", big["L"], ordered, 0.8, 0.8,
+                                          max(args.steps // 4, 3))
+    big0 = v.synth_pair(v.synth_seed(7, 2), 16384, 16384, 0.4, 0.05)
+    res["c2_data_16384"] = case("c2_data_16384", big0["L"], big0["R"], 0.8, 0.8, max(args.steps // 4, 3))
+    for k, r in res.items():
+        ref = res["c2_data_16384"] if k.endswith("16384") else res["c2"]
+        r["slowdown_vs_same_size_benchmark_data"] = r["ms_per_job"] / ref["ms_per_job"]
+    c = res["c2"]
+    return _common("descriptor pairs scored/s", n * n / (c["ms_per_job"] / 1e3), "descriptor pairs/s",
+                   args.steps, 3, c["ms_per_job"] / 1e3, "int8",
+                   {"workload": "survivor-overflow cases at 8192 x 8192 and 16384 x 16384 (see cases)"}) | \
+        {"cases": res}
 
 
 def run_voctree(args, v):

@@ -22,19 +22,21 @@ SCALE = np.float32(1.0 / 262144.0)
 
 
 def survivor_threshold(min_score: float, min_ratio: float) -> int:
-    """Integer survivor threshold (scores in units of 2^-18); -1 keeps all."""
+    """Integer survivor threshold (scores in units of 2^-18); -1 keeps all, 2^28 none."""
     ms = float(np.float32(min_score))
     mr = float(np.float32(min_ratio))
     if mr < 0:
         t = ms
     elif mr > 0:
         t = min(ms, 1.0 - (1.0 - ms) / (mr * (1.0 - 1e-5)))
+    elif mr == 0:
+        t = max(ms, 1.0)  # accepted only with best > 1 (the ratio is then negative)
     else:
-        return -1
+        return 1 << 28  # NaN: nothing is accepted
     if t != t:
-        return -1
+        return 1 << 28
     x = math.floor(t * 262144.0)
-    return int(min(max(x, -1), 2147483000))
+    return int(min(max(x, -1), 1 << 28))
 
 
 def rule_accept(best, nxt, min_score, min_ratio) -> bool:

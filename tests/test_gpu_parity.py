@@ -67,6 +67,27 @@ def test_c1_2048_matches_oracle(m, kernel, rule):
     assert got == oracle_pairs(d["L"], d["R"], *rule)
 
 
+@pytest.mark.parametrize("rule", [(0.9, 0.05), (0.0, 0.0), (0.8, float("nan")), (float("nan"), 0.8)])
+def test_rules_without_a_useful_filter(m, rule):
+    """A tiny ratio (no survivor filter: the automatic choice takes the plain path;
+    the int8 scan, forced, recomputes every overflowed region), minRatio 0 (only bests
+    above 1 pass) and NaN thresholds (nothing passes) -- int8, plain and automatic
+    paths all equal the oracle.  The pair mixes copies scaled up by 1.1 so that some
+    scores exceed 1."""
+    say("filterless rules", rule)
+    d = pair(0, 3, 1024, 1088)
+    L, R = d["L"], d["R"].copy()
+    R[:64] = np.clip(np.rint(L[:64] * 1.1), 0, 255).astype(np.uint8)
+    want = oracle_pairs(L, R, *rule)
+    for kernel in (v.KERNEL_I8, v.KERNEL_PLAIN, v.KERNEL_AUTO):
+        got = run(m, L, R, *rule, kernel=kernel)
+        assert got == want, (rule, kernel)
+        if kernel == v.KERNEL_AUTO:
+            assert m.lastKernel() == (v.KERNEL_PLAIN if rule == (0.9, 0.05) else v.KERNEL_I8)
+    if rule == (0.0, 0.0):
+        assert len(want) > 0
+
+
 SHAPES = [(32, 32), (64, 96), (96, 1000), (1000, 96), (2080, 704), (256, 256), (288, 320), (4096, 128),
           (128, 4096), (320, 2336)]
 

@@ -225,6 +225,85 @@ def test_all_below_threshold_reject():
     assert arg[0] == -1
 
 
+# ---- the GPU paths' survivor filter ----------------------------------------------
+
+def _row_with_score(x, z, lo, hi):
+    """A uint8 row y whose exact score with x, sum(x y) / 2^18, lies in (lo, hi]:
+    y = round(512 w) for w = a x / 512 + b z, bisecting on the blend (scores fall as
+    the weight moves from x to z; a scale above 1 reaches scores above 1)."""
+    xf = x.astype(np.float64) / 512
+    zf = z.astype(np.float64) / 512
+    def score(t):
+        w = np.clip((1 - t) * xf + t * zf, 0, None) * (1.02 if lo >= 0.999 else 1.0) * (1 - t / 50)
+        y = np.clip(np.rint(w * 512), 0, 255).astype(np.uint8)
+        return int(x.astype(np.int64) @ y.astype(np.int64)) / 262144.0, y
+    a, b = 0.0, 1.0
+    for _ in range(60):
+        t = 0.5 * (a + b)
+        sc, y = score(t)
+        if lo < sc <= hi:
+            return y
+        if sc > hi:
+            a = t
+        else:
+            b = t
+    raise AssertionError(f"no row with a score in ({lo}, {hi}]")
+
+
+def _filter_case():
+    """SIFT-like rows, noisy copies, exact duplicates (ties), brighter copies (scores
+    near 1.15), and rows built so that scores sit right at the filters' boundaries: best
+    scores just above and just below 1 (the minRatio 0 rule), and a best just above 0.8
+    with a second best just above 0.75 (the 0.8 / 0.8 rule's T)."""
+    rng = np.random.default_rng(7)
+    d = _pair(77, 300, 320, 0.5, 0.03)
+    L, R = d["L"].copy(), d["R"].copy()
+    R[:30] = L[:30] // 2 + rng.integers(0, 8, (30, 128), dtype=np.uint8)
+    R[30:40] = np.clip(np.rint(L[:10] * 1.15), 0, 255).astype(np.uint8)  # scores near 1.15 with their lefts
+    L[200:210] = L[190:200]
+    z = d["R"][300:320]
+    L[260:268] = _pair(78, 8, 0)["L"]  # fresh rows: no copy of them elsewhere in R
+    for k in range(6):  # bests in (1, 1.002] and (0.998, 1]
+        R[240 + k] = _row_with_score(L[240 + k], z[k], 1.0, 1.002)
+        R[250 + k] = _row_with_score(L[250 + k], z[k + 6], 0.998, 1.0)
+    for k in range(8):  # best in (0.8, 0.8015], second in (0.7501, 0.758] or just below 0.75
+        R[260 + k] = _row_with_score(L[260 + k], z[k], 0.8, 0.8015)
+        R[270 + k] = _row_with_score(L[260 + k], z[k + 8], 0.7501, 0.758) if k < 5 else \
+            _row_with_score(L[260 + k], z[k + 8], 0.745, 0.75)
+    return L, R
+
+
+@pytest.mark.parametrize("rule", [(0.9, 0.8), (0.8, 0.8), (0.5, 0.95), (1.1, 0.8), (0.8, -1.0), (0.0, 0.0),
+                                  (0.8, 0.0), (1.5, 0.0), (0.9, 0.05), (0.8, float("nan")),
+                                  (float("nan"), 0.8)])
+def test_survivor_filter_keeps_every_decision(rule):
+    """The scans drop every score at or below the rule's survivor threshold (the
+    proof at the top of k_i8.hip).  Dropping them -- the oracle's passes on the
+    score matrix with those entries replaced by the passes' -1e30 initial value --
+    must leave every decision as it is, and a threshold one step stronger than the
+    proof allows must not (on the rows built at the boundaries)."""
+    from oracle.learned import survivor_threshold
+    L, R = _filter_case()
+    S = O.scores_u8(L, R).astype(np.int64)  # exact integer scores [right, left]
+    thr = survivor_threshold(*rule)
+
+    def decide(t):
+        kept = np.where(S > t, S / 262144.0, -1e30).astype(np.float32)
+        return O.select_f32(kept, *rule)[2]
+
+    full = O.select_f32((S / 262144.0).astype(np.float32), *rule)[2]
+    np.testing.assert_array_equal(decide(thr), full)
+    assert np.array_equal(full, O.find_u8(L, R, *rule)[2])
+    if rule in ((0.0, 0.0), (0.8, 0.0)):
+        # only rows whose best is above 1 pass; the boundary rows sit on both sides
+        assert np.all(S.max(axis=0)[full >= 0] > 262144) and (full[240:246] >= 0).all()
+        assert not np.array_equal(decide(int(262144 * 1.002)), full)
+    if rule == (0.8, 0.8):
+        assert thr < int(0.75 * 262144) + 1 and not np.array_equal(decide(int(0.758 * 262144)), full)
+    if np.isnan(rule[0]) or np.isnan(rule[1]):
+        assert (full < 0).all() and thr == 1 << 28
+
+
 # ---- invariants --------------------------------------------------------------------
 
 def _pair(seed, nL, nR, rho=0.4, sigma=0.05):

@@ -25,7 +25,13 @@
 //    also accepts: (2-2b)/(2-2s) < (1-minScore)/(1-T) = minRatio (1 - 1e-5),
 //    and the FP32 division adds at most 2^-24 relative error.  If both
 //    survive, the top-2 over survivors equals the top-2 over all scores.
-//  * minRatio == 0 (or NaN): no filtering (t = -1; scores are >= 0).
+//  * minRatio == 0: the ratio (2-2b)/max(0.001, 2-2s) has a positive
+//    denominator whatever s is, so it is below 0 exactly when b > 1; a row is
+//    accepted iff b > max(minScore, 1), which t = floor(max(minScore, 1) 2^18)
+//    keeps, and s plays no part.  minRatio (or minScore) NaN: nothing is
+//    accepted, t = 2^28 (no score passes).
+//  * a positive minRatio whose T is below 0: no filtering (t = -1; scores are
+//    >= 0); the automatic kernel choice then takes the plain path.
 // Only survivors (typically a few per row) reach the exact top-2 update.
 //
 // Exact top-2 per left row and per right row, order independent: each

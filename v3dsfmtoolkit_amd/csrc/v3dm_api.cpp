@@ -281,11 +281,22 @@ int uploadMasks(v3dm_ctx *c)
    return syncStream(c);
 }
 
-int chooseKernel(v3dm_ctx *c, int *kernel)
+// The automatic choice: the int8 scan for u/512 descriptors, else BF16 +
+// FP32 rescoring -- unless the rule admits no survivor filter (a small
+// positive minRatio: every score could be a deciding second best), where
+// every region of the scan would overflow and be recomputed pair by pair;
+// the plain path (score matrix + selection passes, the same list) is then
+// the faster one by far.
+int chooseKernel(v3dm_ctx *c, int *kernel, float minScore, float minRatio)
 {
    const bool q = c->L.quantized && c->R.quantized;
    int k = c->kernelSel;
-   if (k == V3DM_KERNEL_AUTO) k = q ? V3DM_KERNEL_I8 : V3DM_KERNEL_BF16;
+   if (k == V3DM_KERNEL_AUTO)
+   {
+      const bool unfiltered = q ? survivorThreshold(minScore, minRatio) < 0
+                                : survivorThresholdF(minScore, minRatio) == -INFINITY;
+      k = unfiltered ? V3DM_KERNEL_PLAIN : q ? V3DM_KERNEL_I8 : V3DM_KERNEL_BF16;
+   }
    if (k == V3DM_KERNEL_I8 && !q)
       return fail(c, V3DM_ERR_UNSUPPORTED,
                   "int8 kernel requested but the descriptors are not all of the form u/512, u in [0,255]");
@@ -567,7 +578,7 @@ int runMatch(v3dm_ctx *c, float minScore, float minRatio, bool fromFind)
    if (!c->allocated) return fail(c, V3DM_ERR_STATE, "findPutativeMatches: allocate() has not been called");
    HIPCHK(c, hipSetDevice(c->device));
    int kernel = 0;
-   int rc = chooseKernel(c, &kernel);
+   int rc = chooseKernel(c, &kernel, minScore, minRatio);
    if (rc) return rc;
    if (fromFind)
    {

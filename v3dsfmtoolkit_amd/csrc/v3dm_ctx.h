@@ -59,7 +59,10 @@ extern thread_local std::string g_createError;
 inline int roundUp(int n, int m) { return (n + m - 1) / m * m; }
 
 // Survivor threshold in exact integer score units (score = int * 2^-18); see
-// the proof at the top of k_i8.hip.
+// the proof at the top of k_i8.hip.  -1 keeps every score; kThrNone keeps none
+// (no real row's score reaches 2^24, and a padding column's test limit,
+// threshold + 2^30, stays below 2^31).
+constexpr int32_t kThrNone = 1 << 28;
 inline int32_t survivorThreshold(float minScore, float minRatio)
 {
    double t;
@@ -70,17 +73,19 @@ inline int32_t survivorThreshold(float minScore, float minRatio)
       double tr = 1.0 - (1.0 - (double)minScore) / ((double)minRatio * (1.0 - 1e-5));
       t = std::min((double)minScore, tr);
    }
+   else if (minRatio == 0)
+      t = std::max((double)minScore, 1.0); // accepted only with best > 1 (then the ratio is negative)
    else
-      return -1; // minRatio == 0 or NaN: keep every score
-   if (!(t == t)) return -1;
+      return kThrNone; // NaN: the rule accepts nothing
+   if (!(t == t)) return kThrNone; // minScore NaN: likewise
    double x = std::floor(t * 262144.0);
    if (x < -1.0) return -1;
-   if (x > 2147483000.0) return 2147483000;
+   if (x > (double)kThrNone) return kThrNone;
    return (int32_t)x;
 }
 
 // Float form of the survivor threshold (same proof): the largest float <= t,
-// or -inf when nothing may be filtered.
+// -inf when nothing may be filtered, +inf when the rule accepts nothing.
 inline float survivorThresholdF(float minScore, float minRatio)
 {
    double t;
@@ -88,9 +93,11 @@ inline float survivorThresholdF(float minScore, float minRatio)
       t = (double)minScore;
    else if (minRatio > 0)
       t = std::min((double)minScore, 1.0 - (1.0 - (double)minScore) / ((double)minRatio * (1.0 - 1e-5)));
+   else if (minRatio == 0)
+      t = std::max((double)minScore, 1.0);
    else
-      return -INFINITY;
-   if (!(t == t)) return -INFINITY;
+      return INFINITY;
+   if (!(t == t)) return INFINITY;
    float f = (float)t;
    if ((double)f > t) f = std::nextafter(f, -INFINITY);
    return f;
