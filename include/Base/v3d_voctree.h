@@ -25,8 +25,10 @@
 // [0, nLeaves) throw std::runtime_error (the reference indexes out of bounds);
 // a key with a non-finite value gets -1 from getVisualWords (the reference's
 // descent then reads outside the tree); LeafDocuments is movable, not
-// copyable, and has no serialize()/getDocumentHistogram() (no driver uses
-// them).
+// copyable.  LeafDocuments::serialize (with load / save and the stream
+// operators) is there when the reference's Base/v3d_serialization.h is on the
+// include path, as it is in the reference's build: it reads and writes the
+// reference's byte stream.
 #ifndef V3D_VOCTREE_H
 #define V3D_VOCTREE_H
 
@@ -40,6 +42,15 @@
 #include <vector>
 
 #include "../v3dm_voctree.h"
+
+#if defined(__has_include)
+#if __has_include("Base/v3d_serialization.h")
+#include <list>
+
+#include "Base/v3d_serialization.h"
+#define V3DM_HAVE_V3D_SERIALIZATION 1
+#endif
+#endif
 
 namespace V3D
 {
@@ -207,6 +218,20 @@ struct LeafDocuments
       return id;
    }
 
+   // getDocumentHistogram (v3d_voctree.h:166-178): histogram is resized to the
+   // document count (earlier entries kept, as resize keeps them), then every
+   // word of the list adds 1 to each document in its inverted file.
+   void getDocumentHistogram(std::vector<int> const &visualWords, std::vector<int> &histogram) const
+   {
+      int nDocs = 0;
+      v3dm_leaf_documents_count(_docs, &nDocs);
+      histogram.resize((size_t)nDocs, 0);
+      static_assert(sizeof(int) == sizeof(int32_t), "int is 32 bits");
+      if (v3dm_leaf_documents_histogram(_docs, (int)visualWords.size(), wordsOf(visualWords),
+                                        reinterpret_cast<int32_t *>(histogram.data()), nDocs) != V3DM_OK)
+         fail("getDocumentHistogram");
+   }
+
    // getVisualWordHistogram (v3d_voctree.h:180-196): leaf -> count.
    void getVisualWordHistogram(std::vector<int> const &visualWords, std::map<int, int> &histogram) const
    {
@@ -214,26 +239,122 @@ struct LeafDocuments
       for (int w : visualWords) ++histogram[w];
    }
 
-   // compute_L1_Scores (v3d_voctree.h:198-240): (score, document), the best
-   // (smallest) first, nBestDocuments of them (all when negative).
+   // compute_L1_Scores (v3d_voctree.h:198-240): one (score, document) per
+   // document, the best (smallest) first.  The reference sorts only the first
+   // nBestDocuments (partial_sort) and leaves the rest unordered; here all are
+   // sorted, so the first nBestDocuments are the reference's and the size is
+   // the document count, as there.
    void compute_L1_Scores(std::vector<int> const &visualWords, std::vector<std::pair<float, int>> &scores,
                           int nBestDocuments = -1)
    {
-      fetch(visualWords, nBestDocuments, scores, [&](int cap, float *s, int32_t *d, int *count) {
-         return v3dm_leaf_documents_l1_scores(_docs, (int)visualWords.size(), wordsOf(visualWords), nBestDocuments, s,
-                                              d, cap, count);
+      (void)nBestDocuments;
+      fetch(visualWords, -1, scores, [&](int cap, float *s, int32_t *d, int *count) {
+         return v3dm_leaf_documents_l1_scores(_docs, (int)visualWords.size(), wordsOf(visualWords), -1, s, d, cap,
+                                              count);
       });
    }
 
-   // computeBinomialScores (v3d_voctree.h:242-289).
+   // computeBinomialScores (v3d_voctree.h:242-289); ordering as compute_L1_Scores.
    void computeBinomialScores(double const p_match, std::vector<int> const &visualWords,
                               std::vector<std::pair<float, int>> &scores, int nBestDocuments = -1)
    {
-      fetch(visualWords, nBestDocuments, scores, [&](int cap, float *s, int32_t *d, int *count) {
-         return v3dm_leaf_documents_binomial_scores(_docs, p_match, (int)visualWords.size(), wordsOf(visualWords),
-                                                    nBestDocuments, s, d, cap, count);
+      (void)nBestDocuments;
+      fetch(visualWords, -1, scores, [&](int cap, float *s, int32_t *d, int *count) {
+         return v3dm_leaf_documents_binomial_scores(_docs, p_match, (int)visualWords.size(), wordsOf(visualWords), -1,
+                                                    s, d, cap, count);
       });
    }
+
+#if V3DM_HAVE_V3D_SERIALIZATION
+   // The reference's inverted-file entry list and its byte layout
+   // (v3d_voctree.h:106-136): the entry count, then (document, count) pairs
+   // inside one scope.
+   struct entries_type : public std::list<std::pair<int, int> >
+   {
+      template <typename Archive> void serialize(Archive &ar)
+      {
+         unsigned int sz = (unsigned int)this->size();
+         ar & sz;
+         SerializationScope<Archive> scope(ar);
+         if (ar.isLoading())
+         {
+            this->clear();
+            for (unsigned int i = 0; i < sz; ++i)
+            {
+               int doc = 0, count = 0;
+               ar & doc & count;
+               this->push_back(std::make_pair(doc, count));
+            }
+         }
+         else
+            for (auto &e : *this)
+            {
+               int doc = e.first, count = e.second;
+               ar & doc & count;
+            }
+      }
+      V3D_DEFINE_LOAD_SAVE(entries_type);
+   };
+
+   // serialize (v3d_voctree.h:291-298): the document count, the inverted files,
+   // then per document its feature count, distinct-word count and 1.0f /
+   // feature count -- the reference's members in its order and types.
+   template <typename Archive> void serialize(Archive &ar)
+   {
+      int nDocuments = 0;
+      SerializableVector<entries_type> invertedFiles;
+      SerializableVector<int> nDocumentFeatures, nDocumentLeaves;
+      SerializableVector<float> rcpNDocumentFeatures;
+      if (!ar.isLoading())
+      {
+         int nLeaves = 0;
+         int64_t nEntries = 0;
+         v3dm_leaf_documents_sizes(_docs, &nLeaves, &nDocuments, &nEntries);
+         std::vector<int32_t> perLeaf((size_t)nLeaves), entries(2 * (size_t)nEntries + 2);
+         nDocumentFeatures.resize((size_t)nDocuments);
+         nDocumentLeaves.resize((size_t)nDocuments);
+         rcpNDocumentFeatures.resize((size_t)nDocuments);
+         if (v3dm_leaf_documents_export(_docs, perLeaf.data(), entries.data(),
+                                        reinterpret_cast<int32_t *>(nDocumentFeatures.data()),
+                                        reinterpret_cast<int32_t *>(nDocumentLeaves.data()),
+                                        rcpNDocumentFeatures.data()) != V3DM_OK)
+            fail("serialize");
+         invertedFiles.resize((size_t)nLeaves);
+         size_t k = 0;
+         for (int l = 0; l < nLeaves; ++l)
+            for (int j = 0; j < perLeaf[(size_t)l]; ++j, ++k)
+               invertedFiles[(size_t)l].push_back(std::make_pair((int)entries[2 * k], (int)entries[2 * k + 1]));
+      }
+      ar & nDocuments;
+      ar & invertedFiles;
+      ar & nDocumentFeatures;
+      ar & nDocumentLeaves;
+      ar & rcpNDocumentFeatures;
+      if (ar.isLoading())
+      {
+         std::vector<int32_t> perLeaf, entries;
+         for (auto const &list : invertedFiles)
+         {
+            perLeaf.push_back((int32_t)list.size());
+            for (auto const &e : list)
+            {
+               entries.push_back(e.first);
+               entries.push_back(e.second);
+            }
+         }
+         if (invertedFiles.empty() || (int)nDocumentFeatures.size() != nDocuments ||
+             (int)nDocumentLeaves.size() != nDocuments || (int)rcpNDocumentFeatures.size() != nDocuments ||
+             v3dm_leaf_documents_import(_docs, (int)invertedFiles.size(), nDocuments, perLeaf.data(),
+                                        entries.empty() ? nullptr : entries.data(),
+                                        reinterpret_cast<int32_t const *>(nDocumentFeatures.data()),
+                                        reinterpret_cast<int32_t const *>(nDocumentLeaves.data()),
+                                        rcpNDocumentFeatures.data()) != V3DM_OK)
+            fail("serialize");
+      }
+   }
+
+   V3D_DEFINE_LOAD_SAVE(LeafDocuments);
+#endif
 
  protected:
    static int32_t const *wordsOf(std::vector<int> const &w)
@@ -260,6 +381,10 @@ struct LeafDocuments
 
    v3dm_leaf_documents *_docs = nullptr;
 }; // end struct LeafDocuments
+
+#if V3DM_HAVE_V3D_SERIALIZATION
+V3D_DEFINE_IOSTREAM_OPS(LeafDocuments);
+#endif
 
 } // namespace V3D
 

@@ -12,9 +12,13 @@
 //    catch it) instead of exiting the process (v3d_cudamatching.cuh:9-24);
 //  * runGuided*Matching without allocate(..., true) throws instead of using
 //    unallocated buffers;
-//  * the guided members are templates over the matrix / point types, so any
-//    type indexable as F[i][j] and p[0], p[1] works (V3D::Matrix3x3d and
-//    V3D::Vector2f included) without pulling in the V3D math headers;
+//  * the guided members take the reference's V3D::Matrix3x3d / V3D::Vector2f
+//    (exact signatures, when Math/v3d_linear.h is on the include path, as it is
+//    in the reference's build) and, as templates, any type indexable as
+//    F[i][j] and p[0], p[1];
+//  * the protected _d_* members point at this engine's device buffers, which
+//    differ from the reference's: see v3dm_get_device_buffers in v3dmatch.h
+//    (scores exist only after the plain kernel; positions are interleaved);
 //  * the destructor releases the device state (the reference leaves it to
 //    deallocate()).
 //
@@ -38,6 +42,13 @@
 #include "../v3dm_voctree.h"
 #include "../v3dmatch.h"
 
+#if defined(__has_include)
+#if __has_include("Math/v3d_linear.h")
+#include "Math/v3d_linear.h"
+#define V3DM_HAVE_V3D_LINEAR 1
+#endif
+#endif
+
 namespace V3D_CUDA
 {
 
@@ -45,7 +56,8 @@ struct SIFT_Matching
 {
    SIFT_Matching()
       : _warnOnRounding(true), _useReductionPath(false), _useGuidedMatching(false), _nMaxLeftFeatures(0),
-        _nMaxRightFeatures(0), _nLeftFeatures(0), _nRightFeatures(0), _ctx(0)
+        _nMaxRightFeatures(0), _nLeftFeatures(0), _nRightFeatures(0), _d_leftFeatures(0), _d_rightFeatures(0),
+        _d_scores(0), _d_leftPositions(0), _d_rightPositions(0), _ctx(0)
    {
    }
 
@@ -74,6 +86,7 @@ struct SIFT_Matching
       _nMaxLeftFeatures = nMaxLeftFeatures - (nMaxLeftFeatures & 31);
       _nMaxRightFeatures = nMaxRightFeatures - (nMaxRightFeatures & 31);
       _nLeftFeatures = _nRightFeatures = 0;
+      refreshBuffers();
    }
 
    void allocate(int nMaxFeatures, bool useGuidedMatching = false)
@@ -84,6 +97,7 @@ struct SIFT_Matching
    void deallocate()
    {
       if (_ctx) check(v3dm_deallocate(_ctx));
+      refreshBuffers();
    }
 
    void setLeftFeatures(int nFeatures, float const *descriptors)
@@ -119,7 +133,26 @@ struct SIFT_Matching
       int count = 0;
       check(v3dm_find_putative_matches(context(), minScore, minRatio, buf.data(), capacity(), &count));
       unpack(buf, count, matches);
+      refreshBuffers();
    }
+
+#if V3DM_HAVE_V3D_LINEAR
+   void runGuidedMatching(V3D::Matrix3x3d const &F, float const distanceThreshold,
+                          V3D::Vector2f const *leftPositions, V3D::Vector2f const *rightPositions,
+                          std::vector<std::pair<int, int> > &matches, float const minScore = 0.9f,
+                          float const minRatio = 0.8f)
+   {
+      guided(1, F, distanceThreshold, leftPositions, rightPositions, matches, minScore, minRatio);
+   }
+
+   void runGuidedHomographyMatching(V3D::Matrix3x3d const &H, float const distanceThreshold,
+                                    V3D::Vector2f const *leftPositions, V3D::Vector2f const *rightPositions,
+                                    std::vector<std::pair<int, int> > &matches, float const minScore = 0.9f,
+                                    float const minRatio = 0.8f)
+   {
+      guided(2, H, distanceThreshold, leftPositions, rightPositions, matches, minScore, minRatio);
+   }
+#endif
 
    template <class Matrix3x3, class Point2>
    void runGuidedMatching(Matrix3x3 const &F, float const distanceThreshold, Point2 const *leftPositions,
@@ -142,6 +175,10 @@ struct SIFT_Matching
    bool _useReductionPath;
    bool _useGuidedMatching;
    int _nMaxLeftFeatures, _nMaxRightFeatures, _nLeftFeatures, _nRightFeatures;
+   // device buffers (v3d_cudamatching.h:64-65), refreshed after every call; null
+   // where this engine keeps no such buffer (v3dm_get_device_buffers)
+   float *_d_leftFeatures, *_d_rightFeatures, *_d_scores;
+   float *_d_leftPositions, *_d_rightPositions;
    v3dm_ctx *_ctx;
 
  private:
@@ -167,7 +204,22 @@ struct SIFT_Matching
 
    int capacity() const { return _nMaxLeftFeatures > 0 ? _nMaxLeftFeatures : 1; }
 
-   void refreshCounts() { v3dm_get_counts(_ctx, &_nLeftFeatures, &_nRightFeatures); }
+   void refreshCounts()
+   {
+      v3dm_get_counts(_ctx, &_nLeftFeatures, &_nRightFeatures);
+      refreshBuffers();
+   }
+
+   void refreshBuffers()
+   {
+      const float *b[5] = {0, 0, 0, 0, 0};
+      if (_ctx) v3dm_get_device_buffers(_ctx, &b[0], &b[1], &b[2], &b[3], &b[4]);
+      _d_leftFeatures = const_cast<float *>(b[0]);
+      _d_rightFeatures = const_cast<float *>(b[1]);
+      _d_scores = const_cast<float *>(b[2]);
+      _d_leftPositions = const_cast<float *>(b[3]);
+      _d_rightPositions = const_cast<float *>(b[4]);
+   }
 
    static void unpack(std::vector<std::int32_t> const &buf, int count, std::vector<std::pair<int, int> > &matches)
    {
@@ -202,6 +254,7 @@ struct SIFT_Matching
                                                                minRatio, buf.data(), capacity(), &count);
       check(rc);
       unpack(buf, count, matches);
+      refreshBuffers();
    }
 };
 

@@ -90,6 +90,25 @@ int v3dm_leaf_documents_insert(v3dm_leaf_documents *docs, int n, const int32_t *
 
 int v3dm_leaf_documents_count(const v3dm_leaf_documents *docs, int *nDocuments);
 
+/* getDocumentHistogram(visualWords, histogram) (v3d_voctree.h:166-178): for every
+ * word of the list (repeats included), 1 is ADDED to histogram[d] for each
+ * document d in that word's inverted file; histogram holds cap >= documents
+ * entries (the caller sizes and clears it, as the reference's resize does). */
+int v3dm_leaf_documents_histogram(const v3dm_leaf_documents *docs, int n, const int32_t *words, int32_t *histogram,
+                                  int cap);
+
+/* The state serialize() writes (v3d_voctree.h:291-300): leaves, documents and
+ * inverted-file entries, then per leaf its entry count, the (document, count)
+ * entries leaf after leaf in insertion order, and per document its feature
+ * count, distinct-word count and 1.0f / feature count.  Any output may be NULL. */
+int v3dm_leaf_documents_sizes(const v3dm_leaf_documents *docs, int *nLeaves, int *nDocuments, int64_t *nEntries);
+int v3dm_leaf_documents_export(const v3dm_leaf_documents *docs, int32_t *entriesPerLeaf, int32_t *entries,
+                               int32_t *nDocumentFeatures, int32_t *nDocumentLeaves, float *rcpNDocumentFeatures);
+/* Replaces the whole state with the given one (serialize() when loading). */
+int v3dm_leaf_documents_import(v3dm_leaf_documents *docs, int nLeaves, int nDocuments, const int32_t *entriesPerLeaf,
+                               const int32_t *entries, const int32_t *nDocumentFeatures,
+                               const int32_t *nDocumentLeaves, const float *rcpNDocumentFeatures);
+
 /* computeBinomialScores(pMatch, visualWords, scores, nBest) (v3d_voctree.h:242-289):
  * per document the double-precision log-likelihood ratio z0 - z1 of the
  * query's k shared words under p_random = max(n_doc, n_query) / nLeaves and

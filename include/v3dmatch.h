@@ -92,6 +92,22 @@ int v3dm_set_right_u8(v3dm_ctx *ctx, int n, const uint8_t *desc);
 /* Current (rounded, clamped) feature counts. */
 int v3dm_get_counts(const v3dm_ctx *ctx, int *nLeft, int *nRight);
 
+/* Device buffers of the current state, for code that read SIFT_Matching's
+ * protected _d_leftFeatures, _d_rightFeatures, _d_scores, _d_leftPositions and
+ * _d_rightPositions (v3d_cudamatching.h:60-66).  Each output may be NULL, and
+ * receives NULL when this engine holds no such buffer:
+ *  - left / right: the FP32 descriptors (n x 128, row-major); present for float
+ *    input, and for uint8 input once the plain path has run;
+ *  - scores: the nRight x nLeft FP32 score matrix with guided masks applied
+ *    (the reference's layout), present only when the plain path produced the
+ *    last list (the int8 and BF16 scans never store scores);
+ *  - leftPos / rightPos: the first guided call's positions since the last
+ *    findPutativeMatches, interleaved (x, y) (the reference stores all x,
+ *    then all y).
+ * The pointers stay valid until the next call that changes the state. */
+int v3dm_get_device_buffers(const v3dm_ctx *ctx, const float **left, const float **right, const float **scores,
+                            const float **leftPos, const float **rightPos);
+
 /* --- matching ------------------------------------------------------------- */
 
 /* findPutativeMatches(matches, minScore, minRatio) --

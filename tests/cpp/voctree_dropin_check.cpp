@@ -58,10 +58,15 @@ int main(int argc, char **argv)
    {
       const int b = v * perView, e = std::min(n, b + perView);
       std::vector<int> visualWords(host.begin() + b, host.begin() + e);
+      // one score per earlier document; only the first nBest are ordered
       invertedFiles.computeBinomialScores(pMatch, visualWords, scores, nBest);
-      for (auto const &s : scores) std::fprintf(out, "%d %d %.9g\n", v, s.second, s.first);
+      if ((int)scores.size() != v) return 10;
+      for (int k = 0; k < std::min(v, nBest); ++k)
+         std::fprintf(out, "%d %d %.9g\n", v, scores[k].second, scores[k].first);
       invertedFiles.compute_L1_Scores(visualWords, scores, nBest);
-      for (auto const &s : scores) std::fprintf(out, "L1 %d %d %.9g\n", v, s.second, s.first);
+      if ((int)scores.size() != v) return 10;
+      for (int k = 0; k < std::min(v, nBest); ++k)
+         std::fprintf(out, "L1 %d %d %.9g\n", v, scores[k].second, scores[k].first);
       if (invertedFiles.insertDocument(visualWords) != v) return 8;
    }
    std::fclose(out);

@@ -43,3 +43,38 @@ def test_reference_driver_compiles_against_dropin_headers(name, tmp_path):
     if name == "match_using_voctree":
         assert os.path.join(ours, "Base", "v3d_voctree.h") in included
     assert not any(p.startswith(REF) and p.endswith(("v3d_cudamatching.h", "v3d_voctree.h")) for p in included)
+
+
+SUBCLASS = r"""
+#include <utility>
+#include <vector>
+#include "CUDA/v3d_cudamatching.h"
+// a subclass reading the protected device members, and member pointers of the
+// guided calls' exact types (v3d_cudamatching.h:48-66)
+struct Sub : V3D_CUDA::SIFT_Matching
+{
+   float *scores() const { return _d_scores; }
+   float *features() const { return _d_leftFeatures ? _d_leftFeatures : _d_rightFeatures; }
+   float *positions() const { return _d_leftPositions ? _d_leftPositions : _d_rightPositions; }
+   bool guided() const { return _useGuidedMatching; }
+};
+typedef void (V3D_CUDA::SIFT_Matching::*Guided)(V3D::Matrix3x3d const &, float, V3D::Vector2f const *,
+                                                V3D::Vector2f const *, std::vector<std::pair<int, int> > &, float,
+                                                float);
+Guided g1 = &V3D_CUDA::SIFT_Matching::runGuidedMatching;
+Guided g2 = &V3D_CUDA::SIFT_Matching::runGuidedHomographyMatching;
+int main() { Sub s; return s.guided() ? 1 : 0; }
+"""
+
+
+@pytest.mark.skipif(not os.path.isdir(os.path.join(REF, "V3D")), reason="reference tree not present")
+@pytest.mark.parametrize("headers", ["dropin", "reference"])
+def test_subclass_and_member_pointers_compile(headers, tmp_path):
+    """The same code compiles against the reference's header (with its CUDA switch
+    on) and against the drop-in one."""
+    (tmp_path / "sub.cpp").write_text(SUBCLASS)
+    inc = (["-I", os.path.join(ROOT, "include")] if headers == "dropin" else ["-DV3DLIB_ENABLE_CUDA"])
+    cmd = ["g++", "-std=c++17", "-fsyntax-only", "-w", "-fpermissive", *inc, "-I", os.path.join(REF, "V3D", "include"),
+           "sub.cpp"]
+    r = subprocess.run(cmd, cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-3000:]

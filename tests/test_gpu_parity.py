@@ -88,6 +88,30 @@ def test_rules_without_a_useful_filter(m, rule):
         assert len(want) > 0
 
 
+def test_device_buffers_follow_the_path(m):
+    """v3dm_get_device_buffers (the drop-in header's _d_* members): the score
+    matrix exists after the plain path only, FP32 descriptors once the plain path
+    converted them, positions after a guided call."""
+    import ctypes
+    say("device buffers")
+    d = pair(0, 4, 256, 320)
+    ptrs = [ctypes.c_void_p() for _ in range(5)]
+
+    def query():
+        assert m._lib.v3dm_get_device_buffers(m._ctx, *[ctypes.byref(p) for p in ptrs]) == 0
+        return [bool(p.value) for p in ptrs]
+
+    run(m, d["L"], d["R"], 0.8, 0.8, kernel=v.KERNEL_I8)
+    assert query()[2] is False  # the int8 scan stores no scores
+    run(m, d["L"], d["R"], 0.8, 0.8, kernel=v.KERNEL_PLAIN)
+    left, right, scores, lpos, rpos = query()
+    assert left and right and scores and not lpos and not rpos
+    lp = np.zeros((256, 2), np.float32)
+    rp = np.zeros((320, 2), np.float32)
+    m.runGuidedMatching(np.eye(3), 5.0, lp, rp, 0.8, 0.8)
+    assert query()[3:] == [True, True]
+
+
 SHAPES = [(32, 32), (64, 96), (96, 1000), (1000, 96), (2080, 704), (256, 256), (288, 320), (4096, 128),
           (128, 4096), (320, 2336)]
 

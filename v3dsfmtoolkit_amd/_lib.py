@@ -52,6 +52,9 @@ _SIGNATURES = {
     "v3dm_set_left_u8": (ctypes.c_int, [_ctx_p, ctypes.c_int, _u8_p]),
     "v3dm_set_right_u8": (ctypes.c_int, [_ctx_p, ctypes.c_int, _u8_p]),
     "v3dm_get_counts": (ctypes.c_int, [_ctx_p, _c_int_p, _c_int_p]),
+    "v3dm_get_device_buffers": (ctypes.c_int, [_ctx_p, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p),
+                                               ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p),
+                                               ctypes.POINTER(ctypes.c_void_p)]),
     "v3dm_find_putative_matches": (ctypes.c_int, [_ctx_p, ctypes.c_float, ctypes.c_float, _i32_p, ctypes.c_int, _c_int_p]),
     "v3dm_run_guided_matching": (ctypes.c_int, [_ctx_p, _f64_p, ctypes.c_float, _f32_p, _f32_p, ctypes.c_float,
                                                 ctypes.c_float, _i32_p, ctypes.c_int, _c_int_p]),
@@ -159,6 +162,11 @@ _SIGNATURES = {
     "v3dm_leaf_documents_destroy": (None, [ctypes.c_void_p]),
     "v3dm_leaf_documents_insert": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _i32_p, _c_int_p]),
     "v3dm_leaf_documents_count": (ctypes.c_int, [ctypes.c_void_p, _c_int_p]),
+    "v3dm_leaf_documents_histogram": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _i32_p, _i32_p, ctypes.c_int]),
+    "v3dm_leaf_documents_sizes": (ctypes.c_int, [ctypes.c_void_p, _c_int_p, _c_int_p, _i64_p]),
+    "v3dm_leaf_documents_export": (ctypes.c_int, [ctypes.c_void_p, _i32_p, _i32_p, _i32_p, _i32_p, _f32_p]),
+    "v3dm_leaf_documents_import": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _i32_p, _i32_p, _i32_p,
+                                                  _i32_p, _f32_p]),
     "v3dm_leaf_documents_binomial_scores": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, _i32_p,
                                                            ctypes.c_int, _f32_p, _i32_p, ctypes.c_int, _c_int_p]),
     "v3dm_leaf_documents_l1_scores": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _i32_p, ctypes.c_int, _f32_p,

@@ -829,6 +829,19 @@ int v3dm_get_counts(const v3dm_ctx *c, int *nLeft, int *nRight)
    return V3DM_OK;
 }
 
+int v3dm_get_device_buffers(const v3dm_ctx *c, const float **left, const float **right, const float **scores,
+                            const float **leftPos, const float **rightPos)
+{
+   if (!c) return V3DM_ERR_ARG;
+   if (left) *left = c->L.f32Valid ? c->L.dF32 : nullptr;
+   if (right) *right = c->R.f32Valid ? c->R.dF32 : nullptr;
+   if (scores) *scores = c->sValid ? c->dS : nullptr;
+   const bool guided = !c->masks.empty();
+   if (leftPos) *leftPos = guided ? c->L.dPos : nullptr;
+   if (rightPos) *rightPos = guided ? c->R.dPos : nullptr;
+   return V3DM_OK;
+}
+
 int v3dm_find_putative_matches(v3dm_ctx *c, float minScore, float minRatio, int32_t *pairs, int capacity,
                                int *count)
 {

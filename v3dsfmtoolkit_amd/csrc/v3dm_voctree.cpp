@@ -495,6 +495,88 @@ int v3dm_leaf_documents_insert(v3dm_leaf_documents *d, int n, const int32_t *wor
    return V3DM_OK;
 }
 
+int v3dm_leaf_documents_histogram(const v3dm_leaf_documents *d, int n, const int32_t *words, int32_t *histogram,
+                                  int cap)
+{
+   if (!d) return vfail(nullptr, V3DM_ERR_ARG, "leaf_documents_histogram: null object");
+   int rc = checkWords(d->nLeaves, n, words);
+   if (rc) return rc;
+   const int nDocs = (int)d->nDocumentFeatures.size();
+   if (cap < nDocs || (nDocs > 0 && !histogram))
+      return vfail(nullptr, V3DM_ERR_CAPACITY, "leaf_documents_histogram: %d documents, capacity %d", nDocs, cap);
+   for (int q = 0; q < n; ++q)
+      for (const auto &e : d->inverted[(size_t)words[q]]) histogram[(size_t)e.first] += 1;
+   return V3DM_OK;
+}
+
+int v3dm_leaf_documents_sizes(const v3dm_leaf_documents *d, int *nLeaves, int *nDocuments, int64_t *nEntries)
+{
+   if (!d) return vfail(nullptr, V3DM_ERR_ARG, "leaf_documents_sizes: null object");
+   int64_t e = 0;
+   for (const auto &l : d->inverted) e += (int64_t)l.size();
+   if (nLeaves) *nLeaves = d->nLeaves;
+   if (nDocuments) *nDocuments = (int)d->nDocumentFeatures.size();
+   if (nEntries) *nEntries = e;
+   return V3DM_OK;
+}
+
+int v3dm_leaf_documents_export(const v3dm_leaf_documents *d, int32_t *entriesPerLeaf, int32_t *entries,
+                               int32_t *nDocumentFeatures, int32_t *nDocumentLeaves, float *rcpNDocumentFeatures)
+{
+   if (!d) return vfail(nullptr, V3DM_ERR_ARG, "leaf_documents_export: null object");
+   int64_t k = 0;
+   for (size_t l = 0; l < d->inverted.size(); ++l)
+   {
+      if (entriesPerLeaf) entriesPerLeaf[l] = (int32_t)d->inverted[l].size();
+      for (const auto &e : d->inverted[l])
+      {
+         if (entries)
+         {
+            entries[2 * k] = e.first;
+            entries[2 * k + 1] = e.second;
+         }
+         ++k;
+      }
+   }
+   for (size_t i = 0; i < d->nDocumentFeatures.size(); ++i)
+   {
+      if (nDocumentFeatures) nDocumentFeatures[i] = d->nDocumentFeatures[i];
+      if (nDocumentLeaves) nDocumentLeaves[i] = d->nDocumentLeaves[i];
+      if (rcpNDocumentFeatures) rcpNDocumentFeatures[i] = d->rcpNDocumentFeatures[i];
+   }
+   return V3DM_OK;
+}
+
+int v3dm_leaf_documents_import(v3dm_leaf_documents *d, int nLeaves, int nDocuments, const int32_t *entriesPerLeaf,
+                               const int32_t *entries, const int32_t *nDocumentFeatures,
+                               const int32_t *nDocumentLeaves, const float *rcpNDocumentFeatures)
+{
+   if (!d || nLeaves < 1 || nDocuments < 0 || !entriesPerLeaf ||
+       (nDocuments > 0 && (!nDocumentFeatures || !nDocumentLeaves || !rcpNDocumentFeatures)))
+      return vfail(nullptr, V3DM_ERR_ARG, "leaf_documents_import: bad arguments");
+   std::vector<std::vector<std::pair<int, int>>> inv((size_t)nLeaves);
+   int64_t k = 0;
+   for (int l = 0; l < nLeaves; ++l)
+   {
+      if (entriesPerLeaf[l] < 0 || (entriesPerLeaf[l] > 0 && !entries))
+         return vfail(nullptr, V3DM_ERR_ARG, "leaf_documents_import: leaf %d has %d entries", l, entriesPerLeaf[l]);
+      for (int j = 0; j < entriesPerLeaf[l]; ++j, ++k)
+      {
+         const int doc = entries[2 * k], count = entries[2 * k + 1];
+         if (doc < 0 || doc >= nDocuments)
+            return vfail(nullptr, V3DM_ERR_ARG, "leaf_documents_import: leaf %d names document %d of %d", l, doc,
+                         nDocuments);
+         inv[(size_t)l].push_back({doc, count});
+      }
+   }
+   d->nLeaves = nLeaves;
+   d->inverted.swap(inv);
+   d->nDocumentFeatures.assign(nDocumentFeatures, nDocumentFeatures + nDocuments);
+   d->nDocumentLeaves.assign(nDocumentLeaves, nDocumentLeaves + nDocuments);
+   d->rcpNDocumentFeatures.assign(rcpNDocumentFeatures, rcpNDocumentFeatures + nDocuments);
+   return V3DM_OK;
+}
+
 int v3dm_leaf_documents_count(const v3dm_leaf_documents *d, int *nDocuments)
 {
    if (!d || !nDocuments) return vfail(nullptr, V3DM_ERR_ARG, "leaf_documents_count: null argument");
