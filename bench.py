@@ -85,7 +85,7 @@ def main():
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run a plain device copy of this size (scales memory counters)")
     ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5", "voctree", "sequence", "voctree-driver",
-                                           "overflow"],
+                                           "overflow", "mfma-ceiling"],
                     default="c2",
                     help="c2: single pair 8192^2 (default line); c3: 256 uneven pairs in one batch (+ float "
                          "variant); c4: learned path on 64 pairs of 16384^2; c5: single pair 65536^2; voctree: "
@@ -93,7 +93,8 @@ def main():
                          "sequence: the match_sequence counterpart on 64 feature files, radius 3; voctree-driver: "
                          "the match_using_voctree counterpart on the same files; overflow: the survivor-overflow "
                          "cases (same-order near-duplicate frames, identical sets, ratio test off) against the "
-                         "benchmark pair")
+                         "benchmark pair; mfma-ceiling: the int8 scan's step loop alone (no global loads, barriers "
+                         "or survivors) on the benchmark pair's descriptor bytes and on other operand data")
     ap.add_argument("--pairs", type=int, default=0, help="number of pairs for c3 / c4 (0: the workload's own)")
     ap.add_argument("--scan-variant", type=int, default=-1,
                     help="single-pair int8 scan variant (-1: the library measures the fastest)")
@@ -113,7 +114,7 @@ def main():
     if args.workload != "c2":
         out = {"c3": run_c3, "c4": run_c4, "c5": run_c5, "voctree": run_voctree,
                "sequence": run_sequence, "voctree-driver": run_voctree_driver,
-               "overflow": run_overflow}[args.workload](args, v)
+               "overflow": run_overflow, "mfma-ceiling": run_mfma_ceiling}[args.workload](args, v)
         print(json.dumps(out), flush=True)
         return
     run_c2(args, v)
@@ -746,6 +747,41 @@ def run_overflow(args, v):
                    args.steps, 3, c["ms_per_job"] / 1e3, "int8",
                    {"workload": "survivor-overflow cases at 8192 x 8192 and 16384 x 16384 (see cases)"}) | \
         {"cases": res}
+
+
+def run_mfma_ceiling(args, v):
+    """The practical ceiling of the int8 scan's inner loop: its step (8 MFMAs on
+    64 rows per wave, the max / compare test, the column term and the four
+    right fragments from LDS) run alone, two waves per SIMD, on 128 stored
+    rows of the benchmark pair -- and, for comparison, with the fragments kept
+    in registers and on other operand bytes.  The int8 MFMA's clock under its
+    power limit follows the operand values, so the rate depends on the data."""
+    d = v.synth_pair(v.synth_seed(1, 0), 8192, 8192, 0.4, 0.05)
+    rng = np.random.default_rng(7)
+    u = d["L"][:128]
+    data = {
+        "benchmark pair rows (stored u ^ 0x80)": u ^ 0x80,
+        "same rows stored without offset (u; exact only when every u < 128)": np.minimum(u, 127),
+        "uniform random bytes": rng.integers(0, 256, (128, 128), dtype=np.uint8),
+        "constant bytes (u = 5, stored 0x85)": np.full((128, 128), 0x85, np.uint8),
+    }
+    iters = max(2, args.steps) if args.steps != 2000 else 20000
+    variants = {}
+    for name, rows in data.items():
+        for from_lds in (True, False):
+            rates = [v.mfma_ceiling_probe(rows, from_lds, iters) for _ in range(3)]
+            variants[f"{name}; fragments {'from LDS' if from_lds else 'in registers'}"] = {
+                "tops_median": statistics.median(rates), "tops": rates,
+                "frac_of_5000": statistics.median(rates) / 5000.0}
+    head = variants["benchmark pair rows (stored u ^ 0x80); fragments from LDS"]
+    return {"metric": "int8 MFMA rate of the scan's step loop alone", "value": head["tops_median"],
+            "unit": "TOP/s", "n_gpus": 1, "steps": iters, "warmup": 1, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "int8", "data": "synthetic",
+            "config": {"workload": "scan step loop (8 v_mfma_i32_32x32x32_i8 per step, 64 rows per wave in "
+                                   "VGPRs, right fragments from a swizzled LDS stage, max / compare test), "
+                                   "two 4-wave workgroups per CU, operands: 128 rows of the 8192x8192 "
+                                   "benchmark pair"},
+            "frac_of_5000": head["frac_of_5000"], "variants": variants}
 
 
 def run_voctree(args, v):

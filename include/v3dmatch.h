@@ -312,6 +312,18 @@ int v3dm_device_count(int *count);
  * register v of lane l holds row 8*(v/4) + 4*(l/32) + v%4, column l%32. */
 int v3dm_mfma_layout_probe(int device, int32_t *out);
 
+/* Rate of the int8 scan's step loop alone on given data: 8
+ * v_mfma_i32_32x32x32_i8 per step on 64 rows per wave held in registers, the
+ * max / compare test, the column term and (fromLds != 0) the four right
+ * fragments read from LDS in the scan's swizzled layout, two waves per SIMD,
+ * no global loads, barriers or survivors.  rows: 128 rows x 128 bytes in the
+ * scan's stored form (u ^ 0x80), used as the left rows and the LDS stages.
+ * The int8 MFMA's clock under its power limit depends on the operand values,
+ * so this is the practical ceiling of the scan's inner loop on that data.
+ * Writes the rate in TOP/s (one pair = 256 ops) to *topsOut.  Allocates and
+ * synchronises: a diagnostic, not for use beside running jobs. */
+int v3dm_mfma_ceiling_probe(int device, const uint8_t *rows, int fromLds, int iters, double *topsOut);
+
 /* ---- learned proposal path (no counterpart in the reference) -------------
  *
  * An embedding network (128 -> 256 GELU -> 64, L2-normalised, BF16 weights,

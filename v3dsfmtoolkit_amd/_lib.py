@@ -94,6 +94,8 @@ _SIGNATURES = {
     "v3dm_synth_seed": (ctypes.c_uint64, [ctypes.c_int, ctypes.c_int]),
     "v3dm_device_count": (ctypes.c_int, [_c_int_p]),
     "v3dm_mfma_layout_probe": (ctypes.c_int, [ctypes.c_int, _i32_p]),
+    "v3dm_mfma_ceiling_probe": (ctypes.c_int, [ctypes.c_int, _u8_p, ctypes.c_int, ctypes.c_int,
+                                               ctypes.POINTER(ctypes.c_double)]),
     "v3dm_copy_probe": (ctypes.c_int, [ctypes.c_int, ctypes.c_int64, ctypes.c_int]),
     "v3dm_set_scan_variant": (ctypes.c_int, [_ctx_p, ctypes.c_int]),
     "v3dm_get_last_scan_variant": (ctypes.c_int, [_ctx_p, _c_int_p]),
@@ -257,6 +259,18 @@ def mfma_layout_probe(device: int = 0) -> np.ndarray:
     out = np.zeros((64, 16), np.int32)
     check(lib().v3dm_mfma_layout_probe(device, ptr(out, ctypes.c_int32)))
     return out
+
+
+def mfma_ceiling_probe(rows: np.ndarray, from_lds: bool = True, iters: int = 20000, device: int = 0) -> float:
+    """TOP/s of the int8 scan's step loop alone on the given 128 stored rows
+    (uint8 view of the scan's u ^ 0x80 bytes); see v3dm_mfma_ceiling_probe."""
+    r = np.ascontiguousarray(rows, dtype=np.uint8)
+    if r.shape != (128, 128):
+        raise ValueError("mfma_ceiling_probe: rows must be 128 x 128 bytes")
+    out = ctypes.c_double(0.0)
+    check(lib().v3dm_mfma_ceiling_probe(device, ptr(r, ctypes.c_uint8), 1 if from_lds else 0, int(iters),
+                                         ctypes.byref(out)))
+    return out.value
 
 
 def copy_probe(nbytes: int, reps: int = 5, device: int = 0) -> None:

@@ -647,6 +647,83 @@ __global__ void k_mfma_probe(int32_t *out)
    for (int v = 0; v < 16; ++v) out[lane * 16 + v] = D[v];
 }
 
+// Ceiling probe of the scan's step loop: per step 8 v_mfma_i32_32x32x32_i8
+// on the wave's 64 rows held in VGPRs, the previous step's max / compare
+// test, the column term read from LDS and (kFromLds) the four right
+// fragments read from a swizzled 16 KiB LDS stage in the scan's layout --
+// with nothing else (no global loads, barriers or survivors).  The operands
+// are the caller's 128 stored rows (A: this wave's 64 of them; LDS: all 128
+// as four 32-row stages), so the measured rate is the loop's rate on that
+// data: the int8 MFMA's clock under its power limit depends on the operand
+// values.  !kFromLds keeps the fragments of one stage in registers.
+template <bool kFromLds>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_mfma_ceiling(
+   const int8_t *__restrict__ rows, int iters, int32_t *__restrict__ out)
+{
+   __shared__ __attribute__((aligned(16))) int8_t stage[4 * kStageBytes];
+   __shared__ int32_t colTerm[4 * kStepCols];
+   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+   const int lrow = lane & 31, half = lane >> 5;
+   for (int u = tid; u < 4 * kStepCols * 8; u += 256) // 16-byte units: 128 rows x 8
+   {
+      const int row = u >> 3, x = u & 7, st = row / kStepCols, r = row % kStepCols;
+      *reinterpret_cast<v4i *>(stage + st * kStageBytes + (r * 8 + (x ^ swz(r))) * 16) =
+         ld16(rows + (size_t)row * kDim + 16 * x);
+   }
+   if (tid < 4 * kStepCols) colTerm[tid] = -(1 << 20);
+   __syncthreads();
+   v4i A[2][4];
+   v16i C[2];
+#pragma unroll
+   for (int b = 0; b < 2; ++b)
+   {
+      const int8_t *ap = rows + (size_t)((wave * kWaveRows + 32 * b + lrow) & 127) * kDim + 64 * half;
+#pragma unroll
+      for (int s = 0; s < 4; ++s) A[b][s] = ld16(ap + 16 * s);
+#pragma unroll
+      for (int e = 0; e < 16; ++e) C[b][e] = (1 << 20) - kBias;
+   }
+   int fragoff[4];
+#pragma unroll
+   for (int k = 0; k < 4; ++k) fragoff[k] = (lrow * 8 + ((4 * half + k) ^ swz(lrow))) << 4;
+   v4i Bk[4];
+#pragma unroll
+   for (int k = 0; k < 4; ++k) Bk[k] = *reinterpret_cast<const v4i *>(stage + fragoff[k]);
+   v16i s0[2] = {C[0], C[0]}, s1[2] = {C[1], C[1]};
+   int hits = 0;
+   auto step = [&](int it, auto cur) {
+      constexpr int c = decltype(cur)::value, n = c ^ 1;
+      const int8_t *sb = stage + (it & 3) * kStageBytes;
+      v4i B[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+      {
+         if (kFromLds) B[k] = *reinterpret_cast<const v4i *>(sb + fragoff[k]);
+         else B[k] = Bk[k];
+      }
+      if (!kFromLds) asm volatile("" : "+v"(B[0]), "+v"(B[1]), "+v"(B[2]), "+v"(B[3]));
+      s0[n] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[0][0], B[0], C[0], 0, 0, 0);
+      s1[n] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[1][0], B[0], C[1], 0, 0, 0);
+#pragma unroll
+      for (int k = 1; k < 4; ++k)
+      {
+         s0[n] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[0][k], B[k], s0[n], 0, 0, 0);
+         s1[n] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[1][k], B[k], s1[n], 0, 0, 0);
+      }
+      const int cs = colTerm[(it & 3) * kStepCols + lrow];
+      const int m = max(max16(s0[c]), max16(s1[c]));
+      if (__builtin_expect(__any(m > (1 << 30) - cs), 0)) ++hits; // never: every score is far below 2^30
+   };
+   const std::integral_constant<int, 0> K0;
+   const std::integral_constant<int, 1> K1;
+   for (int it = 0; it + 1 < iters; it += 2)
+   {
+      step(it, K0);
+      step(it + 1, K1);
+   }
+   out[(size_t)blockIdx.x * 256 + tid] = hits + s0[0][lane & 15] + s1[1][3];
+}
+
 // ---- launchers ------------------------------------------------------------
 
 // Scan variants: (waves per workgroup, right rows per tile).
@@ -780,6 +857,13 @@ hipError_t launch_advance_epoch(uint32_t *epoch, uint32_t *tileCounter, hipStrea
 hipError_t launch_mfma_probe(int32_t *out, hipStream_t st)
 {
    hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, st, out);
+   return hipGetLastError();
+}
+
+hipError_t launch_mfma_ceiling(bool fromLds, int grid, const int8_t *rows, int iters, int32_t *out, hipStream_t st)
+{
+   if (fromLds) hipLaunchKernelGGL(k_mfma_ceiling<true>, dim3(grid), dim3(256), 0, st, rows, iters, out);
+   else hipLaunchKernelGGL(k_mfma_ceiling<false>, dim3(grid), dim3(256), 0, st, rows, iters, out);
    return hipGetLastError();
 }
 

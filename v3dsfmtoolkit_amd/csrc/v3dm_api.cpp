@@ -1006,6 +1006,43 @@ int v3dm_mfma_layout_probe(int device, int32_t *out)
    return V3DM_OK;
 }
 
+int v3dm_mfma_ceiling_probe(int device, const uint8_t *rows, int fromLds, int iters, double *topsOut)
+{
+   if (!rows || !topsOut || iters < 2)
+      return fail(nullptr, V3DM_ERR_ARG, "mfma_ceiling_probe: rows / topsOut NULL or iters < 2");
+   int n = 0;
+   if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n)
+      return fail(nullptr, V3DM_ERR_DEVICE, "mfma_ceiling_probe: device %d not available", device);
+   hipError_t e = hipSetDevice(device);
+   int cus = 0;
+   if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+   const int grid = 2 * std::max(cus, 1); // two 4-wave workgroups per CU: two waves per SIMD, like the scan
+   int8_t *dRows = nullptr;
+   int32_t *dOut = nullptr;
+   hipEvent_t t0 = nullptr, t1 = nullptr;
+   if (e == hipSuccess) e = hipMalloc((void **)&dRows, 128 * kDim);
+   if (e == hipSuccess) e = hipMalloc((void **)&dOut, sizeof(int32_t) * 256 * (size_t)grid);
+   if (e == hipSuccess) e = hipMemcpy(dRows, rows, 128 * kDim, hipMemcpyHostToDevice);
+   if (e == hipSuccess) e = hipEventCreate(&t0);
+   if (e == hipSuccess) e = hipEventCreate(&t1);
+   if (e == hipSuccess) e = launch_mfma_ceiling(fromLds != 0, grid, dRows, iters, dOut, nullptr); // warm-up
+   if (e == hipSuccess) e = hipEventRecord(t0, nullptr);
+   if (e == hipSuccess) e = launch_mfma_ceiling(fromLds != 0, grid, dRows, iters, dOut, nullptr);
+   if (e == hipSuccess) e = hipEventRecord(t1, nullptr);
+   if (e == hipSuccess) e = hipEventSynchronize(t1);
+   float ms = 0.0f;
+   if (e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
+   if (t0) (void)hipEventDestroy(t0);
+   if (t1) (void)hipEventDestroy(t1);
+   if (dRows) (void)hipFree(dRows);
+   if (dOut) (void)hipFree(dOut);
+   if (e != hipSuccess) return fail(nullptr, V3DM_ERR_DEVICE, "mfma_ceiling_probe: %s", hipGetErrorString(e));
+   // 4 waves x (iters rounded down to even) steps x 8 MFMAs of 32 x 32 x 32 x 2 ops
+   const double ops = (double)grid * 4 * (iters & ~1) * 8 * 65536.0;
+   *topsOut = ms > 0.0f ? ops / ((double)ms * 1e-3) / 1e12 : 0.0;
+   return V3DM_OK;
+}
+
 int v3dm_fetch_left_top2(v3dm_ctx *c, int n, float *best, float *second, int32_t *argBest)
 {
    if (!c) return V3DM_ERR_ARG;

@@ -36,6 +36,7 @@ hipError_t launch_prep_bf16(const float *src, int n, int nPad, uint16_t *dst, ui
 hipError_t launch_advance_epoch(uint32_t *epoch, uint32_t *tileCounter, hipStream_t st);
 hipError_t launch_clear_keys(uint64_t *keys, int64_t n, uint32_t *epoch, uint32_t *done, hipStream_t st);
 hipError_t launch_mfma_probe(int32_t *out, hipStream_t st);
+hipError_t launch_mfma_ceiling(bool fromLds, int grid, const int8_t *rows, int iters, int32_t *out, hipStream_t st);
 hipError_t launch_copy_probe(const void *src, void *dst, int64_t bytes, hipStream_t st);
 hipError_t launch_sgemm_seq(int nL, int nR, const float *Lf, const float *Rf, float *S, hipStream_t st,
                             hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
