@@ -206,6 +206,52 @@ def test_survivors_applied_in_the_scan_or_by_the_insertion_pass(gpu_available, n
         mm.close()
 
 
+@pytest.mark.parametrize("variant", list(range(len(v.scan_variants()))))
+def test_single_pair_every_scan_variant(m, variant):
+    """Every int8 scan variant, forced, on single pairs with ragged shapes
+    (tiles cut at the padded edge of either side; for the whole-tile form,
+    fewer than its 16 stages of rows on a side), with the filter on, with
+    minRatio 0 (every score survives, regions overflow and are recomputed) and
+    with the ratio test off: all equal the oracle."""
+    for nL, nR in [(2080, 704), (320, 2336), (544, 512)]:
+        d = pair(0, 300 + nL % 89, nL, nR)
+        m.setScanVariant(variant)
+        try:
+            for rule in [(0.8, 0.8), (0.0, 0.0), (0.8, -1.0)]:
+                say("scan variant", variant, (nL, nR), rule)
+                got = run(m, d["L"], d["R"], *rule, kernel=v.KERNEL_I8)
+                assert m.lastScanVariant() == variant
+                assert got == oracle_pairs(d["L"], d["R"], *rule)
+        finally:
+            m.setScanVariant(-1)
+
+
+@pytest.mark.parametrize("n,inline", [(8192, 1), (12288, 0)])
+def test_whole_tile_form_resident_and_persistent(gpu_available, n, inline):
+    """The whole-tile scan (one 512 x 512 tile per workgroup, all of it staged
+    in LDS) applies its survivors itself when its grid fits the device at once
+    (8192^2: 256 tiles) and leaves them to the insertion kernel when resident
+    workgroups pull tiles from the queue (12288^2: 576 tiles); both equal the
+    oracle with the filter on and with minRatio 0."""
+    import os
+    variant = len(v.scan_variants()) - 1
+    d = pair(3, 41, n, n)
+    mm = v.SIFTMatching(0)
+    mm.enableRoundingWarning(False)
+    try:
+        mm.allocate(n, n)
+        mm.setScanVariant(variant)
+        for rule in [(0.8, 0.8), (0.0, 0.0)]:
+            say("whole-tile scan", n, rule)
+            got = run(mm, d["L"], d["R"], *rule, kernel=v.KERNEL_I8)
+            assert mm.lastScanVariant() == variant
+            assert mm.lastScanInline() == inline
+            tab = O.find_u8_blocked(d["L"], d["R"], *rule, threads=len(os.sched_getaffinity(0)))[2]
+            assert got == O.table_to_pairs(tab)
+    finally:
+        mm.close()
+
+
 @pytest.mark.parametrize("shape,per_tile", [((4, 256, 256), True), ((4, 256, 512), False)])
 def test_insertion_pass_per_tile_and_per_region(gpu_available, shape, per_tile):
     """The insertion kernel takes one wave per tile above 32768 survivor

@@ -71,6 +71,7 @@ __device__ inline v16f mfma_bf16(const v4i &a, const v4i &b, const v16f &c)
 template <int WAVES, int TILE_N>
 __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(2))) void k_bf16_scan(ScanArgsF a)
 {
+   touch_kernargs(a);
    constexpr int kTileM = WAVES * kWaveRows;
    constexpr int kGroupBF = WAVES >= 8 ? 2 : 1;            // steps per barrier
    constexpr int kGroupBytesBF = kGroupBF * kStageBytesBF;
@@ -308,6 +309,7 @@ __device__ inline void rescore_insert(const ScanArgsF &a, const PairDesc &pd, ui
 // inserted; an overflowed region rescores its whole 64 x tileN block.
 __global__ __launch_bounds__(256) void k_f32_rescore(ScanArgsF a)
 {
+   touch_kernargs(a);
    const int lane = threadIdx.x & 63;
    const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
    if (r >= a.nTiles * a.waves) return;
@@ -346,6 +348,7 @@ __device__ inline float keyf_value(uint64_t k, uint32_t epoch)
 // As k_i8_finalize, on float-score keys.
 __global__ __launch_bounds__(256) void k_f32_finalize(FinalizeArgs a)
 {
+   touch_kernargs(a);
    const int blk = blockIdx.x;
    PairDesc pd = a.pair0;
    if (!a.hasPair0) pd = a.pairs[pair_of_fin(a.finPair, a.pairs, a.nPairs, blk)];

@@ -112,6 +112,77 @@ constexpr int kStageBytes = kStepCols * kDim;      // 4 KiB
 constexpr int kGroup = 4;                          // steps per barrier
 constexpr int kGroupBytes = kGroup * kStageBytes;  // 16 KiB
 
+// Rare path of a scan step: every score of the step above lim goes to the
+// wave's survivor region as (score, row within the wave's 64, column within
+// the tile); padding rows and columns never pass, and 4-register blocks with
+// no passing score are skipped whole.  Entries past cap are counted, not
+// written (the region then overflows and is recomputed).  half: lane / 32;
+// jrel: this lane's column within the tile.
+__device__ __forceinline__ void record_survivors(const v16i &acc0, const v16i &acc1, int lim, int cs, int jrel,
+                                                 int half, int cap, uint64_t *region, int &nsurv)
+{
+   int half4 = 4 * half;
+   asm volatile("" : "+v"(half4));
+#pragma unroll
+   for (int bb = 0; bb < 2; ++bb)
+   {
+      const v16i &acc = bb == 0 ? acc0 : acc1;
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+      {
+         const int mq = max(max(acc[4 * q], acc[4 * q + 1]), max(acc[4 * q + 2], acc[4 * q + 3]));
+         if (!__any(mq > lim)) continue;
+#pragma unroll
+         for (int e = 0; e < 4; ++e)
+         {
+            const int v = 4 * q + e;
+            const bool h = acc[v] > lim;
+            const uint64_t ball = __ballot(h);
+            if (ball == 0) continue;
+            const int irel = 32 * bb + 8 * q + half4 + e;
+            const int slot =
+               nsurv + __builtin_amdgcn_mbcnt_hi((uint32_t)(ball >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)ball, 0));
+            if (h && slot < cap)
+               region[slot] = ((uint64_t)(uint32_t)(acc[v] + cs) << 32) | ((uint32_t)irel << 16) | (uint32_t)jrel;
+            nsurv += __builtin_popcountll(ball);
+         }
+      }
+   }
+}
+
+// Tile end of a scan that applies its own survivors: the wave's region
+// (nsurv > 0 entries; the wave's 64 rows from l0, the tile's columns from c0)
+// is inserted into the top-2 keys, or, when it overflowed, recomputed
+// exactly from the descriptors with u8 dot products.
+__device__ __forceinline__ void apply_region(const ScanArgs &a, const PairDesc &pd, const uint64_t *region, int nsurv,
+                                             int l0, int c0, int tileN, int lane)
+{
+   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+   const uint32_t epoch = *a.epoch;
+   if (nsurv <= a.survCap)
+   {
+      for (int e = lane; e < nsurv; e += 64)
+      {
+         const uint64_t w = region[e];
+         survivor(a, pd, epoch, l0 + (int)((w >> 16) & 0xFFFF), c0 + (int)(w & 0xFFFF), (int32_t)(w >> 32));
+      }
+   }
+   else
+   {
+      const int nRows = min(kWaveRows, pd.nL - l0), nCols = min(tileN, pd.nR - c0);
+      for (int e = lane; e < nRows * nCols; e += 64)
+      {
+         const int il = l0 + e / nCols, jr = c0 + e % nCols;
+         const uint32_t *x = reinterpret_cast<const uint32_t *>(a.L + (size_t)(pd.offL + il) * kDim);
+         const uint32_t *y = reinterpret_cast<const uint32_t *>(a.R + (size_t)(pd.offR + jr) * kDim);
+         uint32_t acc = 0;
+         for (int k = 0; k < kDim / 4; ++k) // stored bytes are u ^ 0x80
+            acc = __builtin_amdgcn_udot4(x[k] ^ 0x80808080u, y[k] ^ 0x80808080u, acc, false);
+         if ((int32_t)acc > a.thr) survivor(a, pd, epoch, il, jr, (int32_t)acc);
+      }
+   }
+}
+
 // Right rows reach LDS by direct-to-LDS loads (global_load_lds_dwordx4: each
 // lane's 16 bytes land at LDS base + 16 * lane), so no register ring and no
 // LDS stores: the lane that loads LDS unit p fetches the global 16 bytes the
@@ -189,6 +260,7 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(2)))
    __shared__ __attribute__((aligned(16))) int32_t sC2[kGroup * kStepCols];
    __shared__ __attribute__((aligned(16))) int32_t sC3[kGroup * kStepCols];
    __shared__ int sNextTile;
+   touch_kernargs(a);
    auto bufB = [&](auto bc) -> int8_t * {
       constexpr int b = decltype(bc)::value;
       if constexpr (b == 0) return sB0;
@@ -355,40 +427,8 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(2)))
          const int m = max(max16(acc0), max16(acc1));
          const int lim = thr - cs; // score > thr  <=>  acc > thr - 128*sum(u_right)
          if (__builtin_expect(__any(m > lim), 0))
-         {
-            // rare: every surviving score of the wave to its region (padding
-            // rows and columns never pass; 4-register blocks with no passing
-            // score skipped whole)
-            const int jrel = (g * kGroup + s) * kStepCols + lrow;
-            int half4 = 4 * half;
-            asm volatile("" : "+v"(half4));
-#pragma unroll
-            for (int bb = 0; bb < 2; ++bb)
-            {
-               const v16i &acc = bb == 0 ? acc0 : acc1;
-#pragma unroll
-               for (int q = 0; q < 4; ++q)
-               {
-                  const int mq = max(max(acc[4 * q], acc[4 * q + 1]), max(acc[4 * q + 2], acc[4 * q + 3]));
-                  if (!__any(mq > lim)) continue;
-#pragma unroll
-                  for (int e = 0; e < 4; ++e)
-                  {
-                     const int v = 4 * q + e;
-                     const bool h = acc[v] > lim;
-                     const uint64_t ball = __ballot(h);
-                     if (ball == 0) continue;
-                     const int irel = 32 * bb + 8 * q + half4 + e;
-                     const int slot = nsurv + __builtin_amdgcn_mbcnt_hi((uint32_t)(ball >> 32),
-                                                                        __builtin_amdgcn_mbcnt_lo((uint32_t)ball, 0));
-                     if (h && slot < a.survCap)
-                        region[slot] = ((uint64_t)(uint32_t)(acc[v] + cs) << 32) | ((uint32_t)irel << 16) |
-                                       (uint32_t)jrel;
-                     nsurv += __builtin_popcountll(ball);
-                  }
-               }
-            }
-         }
+            record_survivors(acc0, acc1, lim, cs, (g * kGroup + s) * kStepCols + lrow, half, a.survCap, region,
+                             nsurv);
       }
    };
    const std::false_type kMore;
@@ -430,34 +470,187 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(2)))
       if (lane == 0) a.survCount[(size_t)tile * kScanWaves + wave] = nsurv;
    }
    else if (nsurv > 0)
-   {
-      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-      const uint32_t epoch = *a.epoch;
-      if (nsurv <= a.survCap)
-      {
-         for (int e = lane; e < nsurv; e += 64)
-         {
-            const uint64_t w = region[e];
-            survivor(a, pd, epoch, l0 + (int)((w >> 16) & 0xFFFF), c0 + (int)(w & 0xFFFF), (int32_t)(w >> 32));
-         }
-      }
-      else
-      {
-         const int nRows = min(kWaveRows, pd.nL - l0), nCols = min(kTileN, pd.nR - c0);
-         for (int e = lane; e < nRows * nCols; e += 64)
-         {
-            const int il = l0 + e / nCols, jr = c0 + e % nCols;
-            const uint32_t *x = reinterpret_cast<const uint32_t *>(a.L + (size_t)(pd.offL + il) * kDim);
-            const uint32_t *y = reinterpret_cast<const uint32_t *>(a.R + (size_t)(pd.offR + jr) * kDim);
-            uint32_t acc = 0;
-            for (int k = 0; k < kDim / 4; ++k) // stored bytes are u ^ 0x80
-               acc = __builtin_amdgcn_udot4(x[k] ^ 0x80808080u, y[k] ^ 0x80808080u, acc, false);
-            if ((int32_t)acc > a.thr) survivor(a, pd, epoch, il, jr, (int32_t)acc);
-         }
-      }
-   }
+      apply_region(a, pd, region, nsurv, l0, c0, kTileN, lane);
    if (!a.persistent) break;
    __syncthreads(); // every wave is done with this tile's buffers (no load of it is in flight)
+   if (tid == 0) sNextTile = (int)nextReq;
+   __syncthreads();
+   tile = sNextTile;
+   }
+}
+
+// Whole-tile form: a workgroup of 8 waves scores a 512 x 512 tile with all of
+// the tile's left and right rows and their row terms staged in LDS at once
+// (132 KiB: one workgroup per CU).  Every load of the tile is issued at its
+// start and waited for once; the 16 steps then run with no barrier between
+// them, so a wave that takes the survivor path holds up no other wave, and no
+// step waits for a later group's rows to arrive.  Both sides use the ring's
+// swizzled 16-byte unit layout (32-row stages), so the left fragments are read
+// from LDS as the right ones are.  Tiles as in k_i8_scan (one per workgroup,
+// or resident workgroups pulling tiles from the queue).
+constexpr int kWholeWaves = 8;
+constexpr int kWholeTile = kWholeWaves * kWaveRows; // 512 rows by 512 columns
+
+template <bool INLINE_INSERT>
+__global__ __launch_bounds__(kWholeWaves * 64) __attribute__((amdgpu_waves_per_eu(2))) void k_i8_scan_whole(
+   ScanArgs a)
+{
+   constexpr int kTileM = kWholeTile, kTileN = kWholeTile;
+   constexpr int kStages = kTileN / kStepCols;                        // 16 stages of 32 rows per side
+   constexpr int kLoads = kStages * kStageBytes / 1024 / kWholeWaves; // 1 KiB loads per wave per side
+   static_assert(kLoads == 8 && kStages % kGroup == 0, "tile geometry");
+   __shared__ __attribute__((aligned(16))) int8_t sR[kStages * kStageBytes];
+   __shared__ __attribute__((aligned(16))) int8_t sL[kStages * kStageBytes];
+   __shared__ __attribute__((aligned(16))) int32_t sRs[kTileN];
+   __shared__ __attribute__((aligned(16))) int32_t sLs[kTileM];
+   __shared__ int sNextTile;
+   touch_kernargs(a);
+
+   const int tid = threadIdx.x;
+   const int lane = tid & 63;
+   const int wave = tid >> 6;
+   const int half = lane >> 5, lrow = lane & 31;
+   // LDS unit p (16 bytes) of either side holds stage p / 256, row r = (p / 8) % 32
+   // of it, chunk (p % 8) ^ swz(r); load j of this wave fills units
+   // (wave * kLoads + j) * 64 + lane, all in stage (wave * kLoads + j) / 4
+   int voff[kLoads];
+#pragma unroll
+   for (int j = 0; j < kLoads; ++j)
+   {
+      const int p = (wave * kLoads + j) * 64 + lane, st = p >> 8, pu = p & 255, r = pu >> 3, x = pu & 7;
+      voff[j] = st * kStageBytes + r * kDim + ((x ^ swz(r)) << 4);
+   }
+   int fragoff[4]; // this lane's fragment units within a stage: row lrow, chunk 4 * half + k
+#pragma unroll
+   for (int k = 0; k < 4; ++k) fragoff[k] = (lrow * 8 + ((4 * half + k) ^ swz(lrow))) << 4;
+
+   int tile = blockIdx.x;
+   while (tile < a.nTiles)
+   {
+   uint32_t nextReq = 0;
+   if (a.persistent && tid == 0) nextReq = atomicAdd(a.tileCounter, 1u) + gridDim.x;
+   PairDesc pd = a.pair0; // single pair: straight from the kernel arguments
+   if (a.nPairs > 1) pd = a.pairs[sgpr(pair_of_tile(a.tilePair, a.pairs, a.nPairs, tile))];
+   else if (a.pairsOnDevice) pd = a.pairs[0];
+   pd.offL = ((int64_t)sgpr((int)(pd.offL >> 32)) << 32) | (uint32_t)sgpr((int)pd.offL);
+   pd.offR = ((int64_t)sgpr((int)(pd.offR >> 32)) << 32) | (uint32_t)sgpr((int)pd.offR);
+   pd.slotL = ((int64_t)sgpr((int)(pd.slotL >> 32)) << 32) | (uint32_t)sgpr((int)pd.slotL);
+   pd.slotR = ((int64_t)sgpr((int)(pd.slotR >> 32)) << 32) | (uint32_t)sgpr((int)pd.slotR);
+   pd.nL = sgpr(pd.nL);
+   pd.nR = sgpr(pd.nR);
+   pd.tilesN = sgpr(pd.tilesN);
+   pd.tileBase = sgpr(pd.tileBase);
+   const int t = tile - pd.tileBase;
+   if (a.pairsOnDevice && t >= sgpr(pd.tilesM) * pd.tilesN) break; // past the job's own grid (never persistent)
+   const int tm = t / pd.tilesN, tn = t - tm * pd.tilesN;
+   const int nLpad = (pd.nL + kRowPad - 1) / kRowPad * kRowPad;
+   const int nRpad = (pd.nR + kRowPad - 1) / kRowPad * kRowPad;
+   const int r0 = sgpr(tm * kTileM), c0 = sgpr(tn * kTileN);
+   const int stagesL = sgpr((min(r0 + kTileM, nLpad) - r0) / kStepCols); // stages present (multiples of 4)
+   const int nsteps = sgpr((min(c0 + kTileN, nRpad) - c0) / kStepCols);
+   const int l0 = sgpr(r0 + wave * kWaveRows);
+   const bool active = l0 < nLpad; // wave-uniform; inactive waves still load and sync
+
+   // every load of the tile (nothing past either side's padded end), one wait
+   {
+      const void *srcL = uniform_ptr(a.L + (size_t)(pd.offL + r0) * kDim);
+      const void *srcR = uniform_ptr(a.R + (size_t)(pd.offR + c0) * kDim);
+      const uint32_t dL = lds_addr(sL), dR = lds_addr(sR);
+#pragma unroll
+      for (int j = 0; j < kLoads; ++j)
+      {
+         const int st = (wave * kLoads + j) >> 2;
+         const uint32_t off = (uint32_t)((wave * kLoads + j) * 1024);
+         if (st < stagesL) lds_dma16(srcL, voff[j], (uint32_t)sgpr((int)(dL + off)));
+         if (st < nsteps) lds_dma16(srcR, voff[j], (uint32_t)sgpr((int)(dR + off)));
+      }
+      // row terms: ints [64 wave, 64 wave + 64) of each side
+      if (64 * wave < stagesL * kStepCols)
+         lds_dma4(uniform_ptr(a.Lsum + pd.offL + r0 + 64 * wave), 4 * lane,
+                  (uint32_t)sgpr((int)(lds_addr(sLs) + 256 * wave)));
+      if (64 * wave < nsteps * kStepCols)
+         lds_dma4(uniform_ptr(a.Rsum + pd.offR + c0 + 64 * wave), 4 * lane,
+                  (uint32_t)sgpr((int)(lds_addr(sRs) + 256 * wave)));
+      vm_wait_barrier<0>();
+   }
+   uint64_t *region = a.surv + ((size_t)tile * kWholeWaves + wave) * a.survCap;
+   int nsurv = 0; // survivors this wave found in this tile (wave-uniform)
+   if (active)
+   {
+      const int32_t thr = a.thr;
+      v4i A[2][4];
+      v16i C[2];
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+      {
+         const int8_t *sa = sL + (2 * wave + b) * kStageBytes; // rows l0 + 32 b ..
+#pragma unroll
+         for (int k = 0; k < 4; ++k) A[b][k] = *reinterpret_cast<const v4i *>(sa + fragoff[k]);
+#pragma unroll
+         for (int q = 0; q < 4; ++q)
+         {
+            const v4i rs = *reinterpret_cast<const v4i *>(sLs + wave * kWaveRows + 32 * b + 8 * q + 4 * half);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) C[b][4 * q + e] = rs[e] - kBias;
+         }
+      }
+      auto mfma_step = [&](const v4i(&Bf)[4], v16i &x0, v16i &x1) {
+         x0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[0][0], Bf[0], C[0], 0, 0, 0);
+         x1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[1][0], Bf[0], C[1], 0, 0, 0);
+#pragma unroll
+         for (int k = 1; k < 4; ++k)
+         {
+            x0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[0][k], Bf[k], x0, 0, 0, 0);
+            x1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[1][k], Bf[k], x1, 0, 0, 0);
+         }
+      };
+      v16i p0, p1; // scores of the step in flight
+      {
+         v4i Bz[4];
+#pragma unroll
+         for (int k = 0; k < 4; ++k) Bz[k] = *reinterpret_cast<const v4i *>(sR + fragoff[k]);
+         mfma_step(Bz, p0, p1);
+      }
+      // steps 4 g .. 4 g + 3; `lastc` says at compile time whether they end the
+      // tile (only the last step issues no next-step MFMAs)
+      auto quad = [&](int g, auto lastc) {
+         constexpr bool kLast = decltype(lastc)::value;
+#pragma unroll
+         for (int s = 0; s < kGroup; ++s)
+         {
+            const int S = g * kGroup + s;
+            const bool more = s + 1 < kGroup || !kLast; // constant once unrolled
+            v4i Bn[4];
+            if (more)
+            {
+#pragma unroll
+               for (int k = 0; k < 4; ++k)
+                  Bn[k] = *reinterpret_cast<const v4i *>(sR + (S + 1) * kStageBytes + fragoff[k]);
+            }
+            const v16i acc0 = p0, acc1 = p1;
+            if (more) mfma_step(Bn, p0, p1);
+            const int cs = sRs[S * kStepCols + lrow];
+            const int m = max(max16(acc0), max16(acc1));
+            const int lim = thr - cs; // score > thr  <=>  acc > thr - 128*sum(u_right)
+            if (__builtin_expect(__any(m > lim), 0))
+               record_survivors(acc0, acc1, lim, cs, S * kStepCols + lrow, half, a.survCap, region, nsurv);
+         }
+      };
+      const std::false_type kMore;
+      const std::true_type kEnd;
+      const int ngroups = nsteps / kGroup; // 1 .. 4
+      int g = 0;
+      for (; g + 1 < ngroups; ++g) quad(g, kMore);
+      quad(g, kEnd);
+   }
+   if constexpr (!INLINE_INSERT)
+   {
+      if (lane == 0) a.survCount[(size_t)tile * kWholeWaves + wave] = nsurv;
+   }
+   else if (nsurv > 0)
+      apply_region(a, pd, region, nsurv, l0, c0, kTileN, lane);
+   if (!a.persistent) break;
+   __syncthreads(); // every wave is done with this tile's LDS (no load of it is in flight)
    if (tid == 0) sNextTile = (int)nextReq;
    __syncthreads();
    tile = sNextTile;
@@ -480,6 +673,7 @@ constexpr int kMaxScanWaves = 8;
 template <bool kPerTile>
 __global__ __launch_bounds__(256) void k_i8_insert(ScanArgs a)
 {
+   touch_kernargs(a);
    constexpr int kSpan = kPerTile ? kMaxScanWaves : 1; // regions a wave may take
    const int lane = threadIdx.x & 63;
    const int wid = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -564,6 +758,7 @@ __device__ inline float key_value(uint64_t k, uint32_t epoch)
 // finish advances the epoch; large grids leave that to k_advance_epoch.
 __global__ __launch_bounds__(256) void k_i8_finalize(FinalizeArgs a)
 {
+   touch_kernargs(a);
    const int blk = blockIdx.x;
    PairDesc pd = a.pair0;
    if (!a.hasPair0) pd = a.pairs[pair_of_fin(a.finPair, a.pairs, a.nPairs, blk)];
@@ -726,7 +921,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 
 // ---- launchers ------------------------------------------------------------
 
-// Scan variants: (waves per workgroup, right rows per tile).
+// Scan variants: (waves per workgroup, right rows per tile); the last one is
+// the whole-tile form (k_i8_scan_whole).
 constexpr int kMaxDevicesCached = 64;
 
 struct ScanVariant
@@ -741,6 +937,7 @@ static const ScanVariant kVariants[] = {
    {4, 256, reinterpret_cast<const void *>(&k_i8_scan<4, 256, false>)},
    {4, 2048, reinterpret_cast<const void *>(&k_i8_scan<4, 2048, false>)},
    {4, 4096, reinterpret_cast<const void *>(&k_i8_scan<4, 4096, false>)},
+   {kWholeWaves, kWholeTile, reinterpret_cast<const void *>(&k_i8_scan_whole<false>)},
 };
 constexpr int kNumVariants = (int)(sizeof(kVariants) / sizeof(kVariants[0]));
 
@@ -794,6 +991,12 @@ hipError_t launch_i8_scan(int variant, int grid, const ScanArgs &a, hipStream_t 
    case 9: hipExtLaunchKernelGGL((k_i8_scan<4, 2048, true>), dim3(grid), dim3(256), 0, st, start, stop, 0, a); break;
    case 10: hipExtLaunchKernelGGL((k_i8_scan<4, 4096, false>), dim3(grid), dim3(256), 0, st, start, stop, 0, a); break;
    case 11: hipExtLaunchKernelGGL((k_i8_scan<4, 4096, true>), dim3(grid), dim3(256), 0, st, start, stop, 0, a); break;
+   case 12:
+      hipExtLaunchKernelGGL((k_i8_scan_whole<false>), dim3(grid), dim3(kWholeWaves * 64), 0, st, start, stop, 0, a);
+      break;
+   case 13:
+      hipExtLaunchKernelGGL((k_i8_scan_whole<true>), dim3(grid), dim3(kWholeWaves * 64), 0, st, start, stop, 0, a);
+      break;
    default: return hipErrorInvalidValue;
    }
    return hipGetLastError();

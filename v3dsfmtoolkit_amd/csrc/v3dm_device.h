@@ -105,6 +105,24 @@ __device__ inline int sgpr(int x) { return __builtin_amdgcn_readfirstlane(x); }
 
 __device__ inline int swz(int r) { return (r ^ (r >> 3)) & 7; }
 
+// A kernel reads its argument block with scalar loads placed where each field
+// is first used, and waits for each group before the branch that needs it: a
+// chain of scalar-cache misses at launch (one per 64-byte line, ~0.5 us each).
+// One scalar load per line of the block, issued together at kernel entry and
+// waited for once, brings every line into the scalar cache so that the later
+// loads hit.
+template <typename T> __device__ __forceinline__ void touch_kernargs(const T &args)
+{
+   static_assert(sizeof(T) <= 512, "argument block of at most 8 lines");
+   const uint32_t *p = reinterpret_cast<const uint32_t *>(&args);
+   constexpr int kLines = (int)((sizeof(T) + 63) / 64);
+   uint32_t w[8];
+#pragma unroll
+   for (int i = 0; i < kLines; ++i) w[i] = p[16 * i];
+#pragma unroll
+   for (int i = 0; i < kLines; ++i) asm volatile("" ::"s"(w[i]));
+}
+
 __device__ inline void lds_barrier()
 {
    // LDS writes of this wave done, then the workgroup barrier.  Written out
