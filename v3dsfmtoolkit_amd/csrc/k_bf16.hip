@@ -320,7 +320,7 @@ __global__ __launch_bounds__(256) void k_f32_rescore(ScanArgsF a)
    const uint64_t w0 = region[lane];
    const int tile = r / a.waves, wave = r - tile * a.waves;
    const int p = pair_of_tile(a.tilePair, a.pairs, a.nPairs, tile);
-   const uint32_t epoch = *a.epoch;
+   const uint32_t epoch = job_epoch(a.epoch, a.epochValue);
    const int cnt = a.survCount[r];
    if (cnt == 0) return;
    const PairDesc pd = a.pairs[p];
@@ -353,7 +353,7 @@ __global__ __launch_bounds__(256) void k_f32_finalize(FinalizeArgs a)
    PairDesc pd = a.pair0;
    if (!a.hasPair0) pd = a.pairs[pair_of_fin(a.finPair, a.pairs, a.nPairs, blk)];
    const int k = (blk - pd.finBase) * 256 + threadIdx.x;
-   const uint32_t epoch = *a.epoch;
+   const uint32_t epoch = job_epoch(a.epoch, a.epochValue);
    if (k < pd.nL)
    {
       int out = -1;

@@ -158,7 +158,7 @@ __device__ __forceinline__ void apply_region(const ScanArgs &a, const PairDesc &
                                              int l0, int c0, int tileN, int lane)
 {
    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-   const uint32_t epoch = *a.epoch;
+   const uint32_t epoch = job_epoch(a.epoch, a.epochValue);
    if (nsurv <= a.survCap)
    {
       for (int e = lane; e < nsurv; e += 64)
@@ -708,7 +708,7 @@ __global__ __launch_bounds__(256) void k_i8_insert(ScanArgs a)
       }
    }
    const int p = a.nPairs > 1 ? pair_of_tile(a.tilePair, a.pairs, a.nPairs, tile) : 0;
-   const uint32_t epoch = *a.epoch;
+   const uint32_t epoch = job_epoch(a.epoch, a.epochValue);
    int any = 0;
 #pragma unroll
    for (int w = 0; w < kSpan; ++w) any |= cnt[w];
@@ -763,7 +763,7 @@ __global__ __launch_bounds__(256) void k_i8_finalize(FinalizeArgs a)
    PairDesc pd = a.pair0;
    if (!a.hasPair0) pd = a.pairs[pair_of_fin(a.finPair, a.pairs, a.nPairs, blk)];
    const int k = (blk - pd.finBase) * 256 + threadIdx.x;
-   const uint32_t epoch = *a.epoch;
+   const uint32_t epoch = job_epoch(a.epoch, a.epochValue);
    if (k < pd.nL)
    {
       // two dependent rounds of loads: the left's keys, then (for a key of

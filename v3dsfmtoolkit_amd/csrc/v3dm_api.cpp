@@ -327,7 +327,8 @@ int enqueueI8Job(v3dm_ctx *c, int v, float minScore, float minRatio, bool marks)
    a.top2L = c->L.dTop2;
    a.top1R = c->R.dTop1;
    a.top2R = c->R.dTop2;
-   a.epoch = c->dEpoch;
+   a.epoch = nullptr; // by value: the host counts single-pair jobs
+   a.epochValue = c->hostEpoch;
    a.masks = c->dMasks;
    a.nMasks = (int)c->masks.size();
    a.surv = c->dSurv;
@@ -363,9 +364,10 @@ int enqueueI8Job(v3dm_ctx *c, int v, float minScore, float minRatio, bool marks)
    f.top2L = c->L.dTop2;
    f.top1R = c->R.dTop1;
    f.top2R = c->R.dTop2;
-   f.epoch = c->dEpoch;
-   f.done = c->dEpoch + 1;
-   f.tileCounter = c->dEpoch + 2;
+   f.epoch = nullptr; // by value, and no last-block epoch advance (single-pair scans are never persistent)
+   f.epochValue = c->hostEpoch;
+   f.done = nullptr;
+   f.tileCounter = nullptr;
    f.minScore = minScore;
    f.minRatio = minRatio;
    f.matchOfLeft = c->dMatch;
@@ -476,7 +478,8 @@ int enqueueBF16(v3dm_ctx *c, float minScore, float minRatio)
    a.top2L = c->L.dTop2;
    a.top1R = c->R.dTop1;
    a.top2R = c->R.dTop2;
-   a.epoch = c->dEpoch;
+   a.epoch = nullptr; // by value: the host counts single-pair jobs
+   a.epochValue = c->hostEpoch;
    a.masks = c->dMasks;
    a.nMasks = (int)c->masks.size();
    a.surv = c->dSurv;
@@ -504,9 +507,10 @@ int enqueueBF16(v3dm_ctx *c, float minScore, float minRatio)
    f.top2L = c->L.dTop2;
    f.top1R = c->R.dTop1;
    f.top2R = c->R.dTop2;
-   f.epoch = c->dEpoch;
-   f.done = c->dEpoch + 1;
-   f.tileCounter = c->dEpoch + 2;
+   f.epoch = nullptr; // by value, and no last-block epoch advance (single-pair scans are never persistent)
+   f.epochValue = c->hostEpoch;
+   f.done = nullptr;
+   f.tileCounter = nullptr;
    f.minScore = minScore;
    f.minRatio = minRatio;
    f.matchOfLeft = c->dMatch;

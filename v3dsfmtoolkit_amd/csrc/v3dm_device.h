@@ -103,6 +103,12 @@ __device__ inline v4i ld16(const int8_t *p)
 
 __device__ inline int sgpr(int x) { return __builtin_amdgcn_readfirstlane(x); }
 
+// The epoch of a job: from device memory (jobs whose kernel arguments are
+// fixed, as in a replayed graph, advance it there) or passed by value (the
+// host counts single-pair jobs, so their kernels need no dependent load of it
+// and no last-block advance).
+__device__ inline uint32_t job_epoch(const uint32_t *p, uint32_t value) { return p ? *p : value; }
+
 __device__ inline int swz(int r) { return (r ^ (r >> 3)) & 7; }
 
 // A kernel reads its argument block with scalar loads placed where each field

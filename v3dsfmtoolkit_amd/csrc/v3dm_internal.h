@@ -88,7 +88,8 @@ struct ScanArgs
    int32_t thr;                 // candidate survives iff exact integer score > thr
    uint64_t *top1L, *top2L;     // per left slot
    uint64_t *top1R, *top2R;     // per right slot
-   const uint32_t *epoch;       // device-resident epoch of the current job
+   const uint32_t *epoch;       // device-resident epoch of the current job; null: epochValue
+   uint32_t epochValue;         // the job's epoch, passed by value (single-pair jobs)
    const Mask *masks;           // guided masks (single-pair jobs only)
    int nMasks;
    uint64_t *surv;              // survivor regions, survCap entries per (tile, wave)
@@ -116,7 +117,8 @@ struct ScanArgsF
    float lim;                // candidate iff BF16 score > lim (= thr - proven BF16 error bound)
    float thr;                // a rescored FP32 score enters the top-2 iff > thr
    uint64_t *top1L, *top2L, *top1R, *top2R;
-   const uint32_t *epoch;
+   const uint32_t *epoch; // null: epochValue
+   uint32_t epochValue;
    const Mask *masks;
    int nMasks;
    uint64_t *surv;
@@ -134,7 +136,8 @@ struct FinalizeArgs
    int nPairs;
    int nBlocks;
    const uint64_t *top1L, *top2L, *top1R, *top2R;
-   uint32_t *epoch;
+   uint32_t *epoch;             // null: epochValue (then done is null too)
+   uint32_t epochValue;
    uint32_t *done;              // block counter of the job's last-block epoch advance; null: none
    uint32_t *tileCounter;       // persistent scan's queue head, reset at the end of the job
    float minScore, minRatio;

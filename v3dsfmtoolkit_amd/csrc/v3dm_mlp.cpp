@@ -651,6 +651,10 @@ int v3dm_learned_find(v3dm_ctx *c, v3dm_mlp *m, int topK, float margin, float mi
    }
    const PairDesc &pd = c->hPairUploaded;
    int64_t nFlag[2] = {0, 0};
+   // this job's kernels read the epoch from device memory (its finalize
+   // advances it there); the host's count is the current one (single-pair
+   // int8 and float jobs pass it by value and leave device memory alone)
+   HIPCHK(c, launch_fill_i32(1, (int32_t)c->hostEpoch, reinterpret_cast<int32_t *>(c->dEpoch), c->stream));
    if (nL > 0 && nR > 0)
    {
       hipStream_t st = c->stream;
