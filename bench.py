@@ -321,9 +321,12 @@ def run_extra(args, v):
 
 def _scan_name(v, variant, inline=False):
     """Kernel name of a scan variant as rocprofv3 prints it, e.g. k_i8_scan<8, 512, true>
-    (the last argument: the scan applies its survivors itself)."""
+    or k_i8_scan_whole<true> (the last argument: the scan applies its survivors itself)."""
+    flag = "true" if inline else "false"
+    if v.scan_variant_whole(variant):
+        return f"k_i8_scan_whole<{flag}>"
     w, _, tn = v.scan_variants()[variant]
-    return f"k_i8_scan<{w}, {tn}, {'true' if inline else 'false'}>"
+    return f"k_i8_scan<{w}, {tn}, {flag}>"
 
 
 def _common(metric, value, unit, steps, warmup, wall, dtype, config, higher=True):

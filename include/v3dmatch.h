@@ -286,6 +286,12 @@ int v3dm_scan_variant_count(void);
  * (BF16) form; a float batch accepts only those. */
 int v3dm_scan_variant_count_f32(void);
 int v3dm_scan_variant_info(int variant, int *waves, int *tileRows, int *tileCols);
+/* *wholeTile = 1 when the variant stages its whole tile (both sides' rows) in
+ * LDS at the tile's start and runs its steps with no barrier between them
+ * (kernel k_i8_scan_whole; one workgroup per CU), 0 for the ring form
+ * (k_i8_scan: right rows through a four-group LDS ring, left rows in
+ * registers). */
+int v3dm_scan_variant_form(int variant, int *wholeTile);
 
 /* --- synthetic SIFT-like descriptors (host) -------------------------------- */
 

@@ -94,6 +94,7 @@ _SIGNATURES = {
     "v3dm_synth_seed": (ctypes.c_uint64, [ctypes.c_int, ctypes.c_int]),
     "v3dm_device_count": (ctypes.c_int, [_c_int_p]),
     "v3dm_mfma_layout_probe": (ctypes.c_int, [ctypes.c_int, _i32_p]),
+    "v3dm_scan_variant_form": (ctypes.c_int, [ctypes.c_int, _i32_p]),
     "v3dm_mfma_ceiling_probe": (ctypes.c_int, [ctypes.c_int, _u8_p, ctypes.c_int, ctypes.c_int,
                                                ctypes.POINTER(ctypes.c_double)]),
     "v3dm_copy_probe": (ctypes.c_int, [ctypes.c_int, ctypes.c_int64, ctypes.c_int]),
@@ -281,6 +282,13 @@ def copy_probe(nbytes: int, reps: int = 5, device: int = 0) -> None:
 def scan_variants_f32() -> int:
     """Number of scan variants (the first ones) that also run float descriptors."""
     return int(lib().v3dm_scan_variant_count_f32())
+
+
+def scan_variant_whole(variant: int) -> bool:
+    """True for the whole-tile scan form (k_i8_scan_whole), False for the ring form."""
+    w = ctypes.c_int32(0)
+    check(lib().v3dm_scan_variant_form(variant, ctypes.byref(w)))
+    return bool(w.value)
 
 
 def scan_variants() -> list[tuple[int, int, int]]:

@@ -943,6 +943,7 @@ constexpr int kNumVariants = (int)(sizeof(kVariants) / sizeof(kVariants[0]));
 
 int scan_variant_count() { return kNumVariants; }
 int scan_variant_waves(int v) { return kVariants[v].waves; }
+int scan_variant_whole(int v) { return kVariants[v].fn == reinterpret_cast<const void *>(&k_i8_scan_whole<false>); }
 int scan_variant_tile_m(int v) { return kVariants[v].waves * kWaveRows; }
 int scan_variant_tile_n(int v) { return kVariants[v].tileN; }
 // Survivor entries per (tile, wave) region: 64 per 512 right rows, so a

@@ -841,6 +841,13 @@ int v3dm_scan_variant_info(int variant, int *waves, int *tileRows, int *tileCols
    return V3DM_OK;
 }
 
+int v3dm_scan_variant_form(int variant, int *wholeTile)
+{
+   if (variant < 0 || variant >= scan_variant_count() || !wholeTile) return V3DM_ERR_ARG;
+   *wholeTile = scan_variant_whole(variant);
+   return V3DM_OK;
+}
+
 int v3dm_scan_variant_count(void) { return scan_variant_count(); }
 int v3dm_scan_variant_count_f32(void) { return bf16_scan_variant_count(); }
 

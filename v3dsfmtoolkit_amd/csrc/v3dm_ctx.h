@@ -18,6 +18,7 @@ hipError_t launch_i8_scan(int variant, int grid, const ScanArgs &a, hipStream_t 
                           hipEvent_t stop = nullptr);
 int scan_variant_count();
 int scan_variant_waves(int v);
+int scan_variant_whole(int v); // 1: k_i8_scan_whole (whole tile staged in LDS)
 int scan_variant_tile_m(int v);
 int scan_variant_tile_n(int v);
 int scan_variant_surv_cap(int v);
