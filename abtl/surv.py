@@ -1,0 +1,23 @@
+import sys, os, statistics
+sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+import v3dsfmtoolkit_amd as v
+n = 8192
+d = v.synth_pair(v.synth_seed(1, 0), n, n, 0.4, 0.05)
+m = v.SIFTMatching(0)
+m.enableRoundingWarning(False)
+m.allocate(n, n)
+m.setLeftFeatures(n, d["L"]); m.setRightFeatures(n, d["R"])
+m.setKernel(v.KERNEL_I8)
+whole = [i for i in range(len(v.scan_variants())) if v.scan_variant_whole(i)][0]
+for variant in (whole, 2):
+    m.setScanVariant(variant)
+    for rule in [(0.8, 0.8), (1.5, 0.8), (0.8, 0.8), (1.5, 0.8)]:
+        for _ in range(50): m.findAsync(*rule)
+        m.synchronize()
+        m.profileEnable(True)
+        for _ in range(500): m.findAsync(*rule)
+        m.synchronize()
+        sc, sel, jobs = m.profileRead()
+        m.profileEnable(False)
+        print(f"variant {variant} rule {rule}: scan {sc / jobs * 1e3:.2f} us, finalize {sel / jobs * 1e3:.2f} us", flush=True)
+m.close()

@@ -151,13 +151,15 @@ __device__ __forceinline__ void record_survivors(const v16i &acc0, const v16i &a
 }
 
 // Tile end of a scan that applies its own survivors: the wave's region
-// (nsurv > 0 entries; the wave's 64 rows from l0, the tile's columns from c0)
-// is inserted into the top-2 keys, or, when it overflowed, recomputed
-// exactly from the descriptors with u8 dot products.
+// (nsurv > 0 entries; the wave's 64 rows from l0, the tile's columns from c0;
+// in global memory, or in LDS for kRegionInLds) is inserted into the top-2
+// keys, or, when it overflowed, recomputed exactly from the descriptors with
+// u8 dot products.
+template <bool kRegionInLds = false>
 __device__ __forceinline__ void apply_region(const ScanArgs &a, const PairDesc &pd, const uint64_t *region, int nsurv,
                                              int l0, int c0, int tileN, int lane)
 {
-   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+   if constexpr (!kRegionInLds) asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); // the region's stores
    const uint32_t epoch = job_epoch(a.epoch, a.epochValue);
    if (nsurv <= a.survCap)
    {
@@ -503,6 +505,11 @@ __global__ __launch_bounds__(kWholeWaves * 64) __attribute__((amdgpu_waves_per_e
    __shared__ __attribute__((aligned(16))) int8_t sL[kStages * kStageBytes];
    __shared__ __attribute__((aligned(16))) int32_t sRs[kTileN];
    __shared__ __attribute__((aligned(16))) int32_t sLs[kTileM];
+   // survivor regions of a scan that applies its own survivors: kept in LDS
+   // (the tile end reads them back at LDS latency); otherwise in global memory
+   // for k_i8_insert
+   constexpr int kLdsSurvCap = 64;
+   __shared__ __attribute__((aligned(16))) uint64_t sSurv[INLINE_INSERT ? kWholeWaves : 1][kLdsSurvCap];
    __shared__ int sNextTile;
    touch_kernargs(a);
 
@@ -573,7 +580,9 @@ __global__ __launch_bounds__(kWholeWaves * 64) __attribute__((amdgpu_waves_per_e
                   (uint32_t)sgpr((int)(lds_addr(sRs) + 256 * wave)));
       vm_wait_barrier<0>();
    }
-   uint64_t *region = a.surv + ((size_t)tile * kWholeWaves + wave) * a.survCap;
+   uint64_t *region = INLINE_INSERT ? &sSurv[INLINE_INSERT ? wave : 0][0]
+                                    : a.surv + ((size_t)tile * kWholeWaves + wave) * a.survCap;
+   const int survCap = INLINE_INSERT ? min(a.survCap, kLdsSurvCap) : a.survCap;
    int nsurv = 0; // survivors this wave found in this tile (wave-uniform)
    if (active)
    {
@@ -633,7 +642,7 @@ __global__ __launch_bounds__(kWholeWaves * 64) __attribute__((amdgpu_waves_per_e
             const int m = max(max16(acc0), max16(acc1));
             const int lim = thr - cs; // score > thr  <=>  acc > thr - 128*sum(u_right)
             if (__builtin_expect(__any(m > lim), 0))
-               record_survivors(acc0, acc1, lim, cs, S * kStepCols + lrow, half, a.survCap, region, nsurv);
+               record_survivors(acc0, acc1, lim, cs, S * kStepCols + lrow, half, survCap, region, nsurv);
          }
       };
       const std::false_type kMore;
@@ -648,7 +657,11 @@ __global__ __launch_bounds__(kWholeWaves * 64) __attribute__((amdgpu_waves_per_e
       if (lane == 0) a.survCount[(size_t)tile * kWholeWaves + wave] = nsurv;
    }
    else if (nsurv > 0)
-      apply_region(a, pd, region, nsurv, l0, c0, kTileN, lane);
+   {
+      ScanArgs b = a;
+      b.survCap = survCap;
+      apply_region<true>(b, pd, region, nsurv, l0, c0, kTileN, lane);
+   }
    if (!a.persistent) break;
    __syncthreads(); // every wave is done with this tile's LDS (no load of it is in flight)
    if (tid == 0) sNextTile = (int)nextReq;
