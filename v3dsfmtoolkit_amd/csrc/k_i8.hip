@@ -97,13 +97,15 @@ __device__ inline int max16(const v16i &x)
 //
 // Tiling.  A workgroup of 4 (or 8) waves owns 256 (512) left rows, 64 per
 // wave held in VGPRs as MFMA A operands for the whole tile, and sweeps
-// TILE_N right rows in steps of 32; two workgroups share a CU (256 VGPRs
-// each), so one computes while the other waits at a barrier.  The right rows
-// are staged through LDS once per workgroup and read by all its waves, in
-// groups of 4 steps (16 KiB) with one barrier per group.  Within a group the
-// waves run free, and each wave reads the fragments of step s+1 and issues
-// its MFMAs before it tests the scores of step s (two accumulator sets), so
-// the test overlaps the matrix pipeline.
+// TILE_N right rows in steps of 32; two waves share each SIMD (256 VGPRs
+// each), so one computes while the other waits.  Ring form (k_i8_scan): the
+// right rows are staged through LDS once per workgroup and read by all its
+// waves, in groups of 4 steps (16 KiB) with one barrier per group; within a
+// group the waves run free.  Whole-tile form (k_i8_scan_whole): both sides of
+// a 512 x 512 tile are staged at once and the steps run with no barrier.  In
+// both, each wave reads the fragments of step s+1 and issues its MFMAs before
+// it tests the scores of step s (two accumulator sets), so the test overlaps
+// the matrix pipeline.
 // LDS unit layout: 16-byte unit p = row * 8 + (chunk ^ swz(row)),
 // swz(r) = (r ^ r >> 3) & 7, so the 32 rows of one fragment read (same
 // chunk) fall in distinct bank groups for 8- and 16-lane LDS service.
