@@ -47,15 +47,18 @@ def cpu_baseline(L, R, min_score, min_ratio, budget_s=20.0):
             "seconds_per_pair": med}, tab
 
 
-TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_i8_scan_8192x8192_traffic.md")  # per scan variant
+# committed rocprofv3 memory-counter summaries, one per workload, per scan variant
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_i8_scan_8192x8192_traffic.md")
+TRAFFIC_SUMMARY_BATCH = os.path.join(ROOT, "profiles", "k_i8_scan_batched_256pairs_traffic.md")
+TRAFFIC_SUMMARY_65536 = os.path.join(ROOT, "profiles", "k_i8_scan_65536x65536_traffic.md")
 
 
-def measured_traffic(kernel_name):
-    """HBM bytes per launch of `kernel_name` at the default workload, as the
-    committed rocprofv3 counter summary states it (a line
+def measured_traffic(kernel_name, summary=TRAFFIC_SUMMARY):
+    """HBM bytes per launch of `kernel_name` on the workload `summary` covers, as
+    that committed rocprofv3 counter summary states it (a line
     `traffic_bytes_per_launch <kernel> = <bytes>`), or None."""
     try:
-        with open(TRAFFIC_SUMMARY) as f:
+        with open(summary) as f:
             for line in f:
                 line = line.strip().strip("`")
                 if line.startswith(f"traffic_bytes_per_launch {kernel_name} ="):
@@ -115,6 +118,8 @@ def main():
         out = {"c3": run_c3, "c4": run_c4, "c5": run_c5, "voctree": run_voctree,
                "sequence": run_sequence, "voctree-driver": run_voctree_driver,
                "overflow": run_overflow, "mfma-ceiling": run_mfma_ceiling}[args.workload](args, v)
+        if args.copy_probe_mib > 0:
+            copy_probe(args.copy_probe_mib)
         print(json.dumps(out), flush=True)
         return
     run_c2(args, v)
@@ -411,9 +416,12 @@ def run_c3(args, v):
                    "tile": [tm, tn], "waves": w, "graph": True})
     out["image_pairs_per_s"] = n_pairs / wall
     out["autotune_ms"] = [float(x) for x in tune_ms]
-    out["roofline"] = {"bound": "mfma", "kernel": _scan_name(v, best, False) + " (persistent, batched)",
+    scan_kernel = _scan_name(v, best, False)
+    out["roofline"] = {"bound": "mfma", "kernel": scan_kernel + " (persistent, batched)",
                        "achieved": achieved,
-                       "peak": INT8_PEAK_TOPS, "unit": "TOP/s", "frac": achieved / INT8_PEAK_TOPS, "traffic": None,
+                       "peak": INT8_PEAK_TOPS, "unit": "TOP/s", "frac": achieved / INT8_PEAK_TOPS,
+                       "traffic": (measured_traffic(scan_kernel, TRAFFIC_SUMMARY_BATCH)
+                                   if n_pairs == 256 else None),
                        "launch_avg_us": scan_avg * 1e6}
     out["cpu_baseline"] = {"value": cpu_pairs / cpu_t, "unit": "descriptor pairs scored/s", "cores": threads,
                            "kind": "port", "sample": f"{len(sample)} of the pairs ({cpu_pairs} descriptor pairs), "
@@ -614,9 +622,11 @@ def run_c5(args, v):
     out = _common("descriptor pairs scored/s", n * n / dt_i8, "descriptor pairs/s", 10, 1, dt_i8, "int8",
                   {"workload": f"single pair {n}x{n} uint8, 30% noisy copies (sigma 0.08), minScore 0.8, "
                                "minRatio 0.8, mutual", "n_left": n, "n_right": n, "matches": len(exact)})
-    out["roofline"] = {"bound": "mfma", "kernel": _scan_name(v, scan_var, scan_inline), "achieved": achieved,
+    scan_kernel = _scan_name(v, scan_var, scan_inline)
+    out["roofline"] = {"bound": "mfma", "kernel": scan_kernel, "achieved": achieved,
                        "peak": INT8_PEAK_TOPS,
-                       "unit": "TOP/s", "frac": achieved / INT8_PEAK_TOPS, "traffic": None,
+                       "unit": "TOP/s", "frac": achieved / INT8_PEAK_TOPS,
+                       "traffic": measured_traffic(scan_kernel, TRAFFIC_SUMMARY_65536),
                        "launch_avg_us": scan_avg * 1e6}
     out["learned_top32"] = {"ms_per_pair": dt_l * 1e3, "effective_pairs_per_s": n * n / dt_l,
                             "recall": hit / max(len(e), 1), "precision": hit / max(len(g), 1),
