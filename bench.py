@@ -68,6 +68,80 @@ def measured_traffic(kernel_name, summary=TRAFFIC_SUMMARY):
     return None
 
 
+def traffic_child(args, v):
+    """Counter-pass child of live_traffic: the default pair's job with the scan
+    variant the parent ran, 20 times, then the 256 MiB copy kernel whose
+    counter readings scale the scan's."""
+    n = args.n
+    d = v.synth_pair(v.synth_seed(1, 0), n, n, 0.4, 0.05)
+    m = v.SIFTMatching(0)
+    m.enableRoundingWarning(False)
+    m.allocate(n, n)
+    m.setLeftFeatures(n, d["L"])
+    m.setRightFeatures(n, d["R"])
+    m.setKernel(v.KERNEL_I8)
+    m.setScanVariant(args.scan_variant)
+    for _ in range(20):
+        m.findAsync(0.8, 0.8)
+    m.synchronize()
+    m.close()
+    copy_probe(256)
+
+
+def _short_kernel(name):
+    name = name.strip()
+    if name.startswith("void "):
+        name = name[5:]
+    if name.startswith("v3dm::"):
+        name = name[6:]
+    return name.split("(")[0]
+
+
+def live_traffic(kernel_name, n, variant, timeout_s=150):
+    """HBM bytes per launch of `kernel_name` measured now: two rocprofv3
+    counter passes (FETCH_SIZE, WRITE_SIZE) over a child process that repeats
+    this pair's job with the same scan variant and then runs the 256 MiB copy
+    kernel, each counter scaled by what it reads for that copy.  None if
+    rocprofv3 or the child is unavailable or fails."""
+    import csv
+    import glob
+    import shutil
+    import subprocess
+    import tempfile
+
+    if shutil.which("rocprofv3") is None:
+        return None
+    per = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        tmp = tempfile.mkdtemp(prefix="v3dm_traffic_", dir="/tmp")
+        try:
+            cmd = ["rocprofv3", "--pmc", counter, "--kernel-include-regex", "k_i8_scan|k_copy_probe",
+                   "--output-format", "csv", "-d", tmp, "-o", "t", "--", sys.executable,
+                   os.path.join(ROOT, "bench.py"), "--traffic-child", "--n", str(n), "--scan-variant", str(variant)]
+            r = subprocess.run(cmd, cwd="/tmp", env=dict(os.environ, TMPDIR="/tmp"), timeout=timeout_s,
+                               stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+            files = glob.glob(os.path.join(tmp, "**", "*counter_collection.csv"), recursive=True)
+            if r.returncode != 0 or not files:
+                return None
+            scan, copy = [], []
+            with open(files[0]) as f:
+                for row in csv.DictReader(f):
+                    k = _short_kernel(row["Kernel_Name"])
+                    if k == kernel_name:
+                        scan.append(float(row["Counter_Value"]))
+                    elif k == "k_copy_probe":
+                        copy.append(float(row["Counter_Value"]))
+            if not scan or not copy or statistics.median(copy) <= 0:
+                return None
+            # the copy kernel reads and writes exactly 256 MiB = 262144 KiB per launch
+            per[counter] = statistics.median(scan) * 262144.0 / statistics.median(copy) * 1024.0
+        except (OSError, subprocess.SubprocessError, KeyError, ValueError):
+            return None
+        finally:
+            shutil.rmtree(tmp, ignore_errors=True)
+    return per["FETCH_SIZE"] + per["WRITE_SIZE"]
+
+
 def copy_probe(mib, reps=5):
     """A plain device copy of known size (k_copy_probe: reads and writes
     exactly `mib` MiB per launch), run in the same counter pass as the matcher
@@ -103,6 +177,10 @@ def main():
                     help="single-pair int8 scan variant (-1: the library measures the fastest)")
     ap.add_argument("--profile-variants", action="store_true",
                     help="after the measurement, run 20 jobs with each scan variant (for profiler passes)")
+    ap.add_argument("--traffic", choices=["live", "summary"], default="live",
+                    help="c2: HBM traffic of the scan measured in this run by two rocprofv3 counter passes "
+                         "(live; the committed summary if they cannot run) or read from the summary only")
+    ap.add_argument("--traffic-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--no-extra", action="store_true",
                     help="c2 only: skip the other workloads' summaries (batch, learned path, 65536^2)")
     args = ap.parse_args()
@@ -114,6 +192,9 @@ def main():
 
     import v3dsfmtoolkit_amd as v
 
+    if args.traffic_child:
+        traffic_child(args, v)
+        return
     if args.workload != "c2":
         out = {"c3": run_c3, "c4": run_c4, "c5": run_c5, "voctree": run_voctree,
                "sequence": run_sequence, "voctree-driver": run_voctree_driver,
@@ -217,11 +298,14 @@ def run_c2(args, v):
     sel_avg_s = sel_ms / 1e3 / max(jobs, 1)
     if kernel == v.KERNEL_I8:
         achieved = pairs * 256 / scan_avg_s / 1e12
-        scan_kernel = _scan_name(v, m.lastScanVariant(), m.lastScanInline() == 1)
+        measured_variant = m.lastScanVariant()
+        scan_kernel = _scan_name(v, measured_variant, m.lastScanInline() == 1)
+        summary_traffic = measured_traffic(scan_kernel) if n == 8192 else None
         roof = {"bound": "mfma", "kernel": scan_kernel, "achieved": achieved,
                 "peak": INT8_PEAK_TOPS,
                 "unit": "TOP/s", "frac": achieved / INT8_PEAK_TOPS,
-                "traffic": measured_traffic(scan_kernel) if n == 8192 else None,
+                "traffic": summary_traffic,
+                "traffic_source": "committed rocprofv3 summary" if summary_traffic is not None else None,
                 "ops_per_pair": 256, "launch_avg_us": scan_avg_s * 1e6, "peak_unit": "int8 dense MFMA, TOP/s",
                 # the same work over the whole step (every kernel of the job, launch gaps included)
                 "job_frac": pairs * 256 / wall / 1e12 / INT8_PEAK_TOPS}
@@ -278,6 +362,17 @@ def run_c2(args, v):
     m.close()
     if args.copy_probe_mib > 0:
         copy_probe(args.copy_probe_mib)
+    # HBM traffic of this run's scan kernel, measured now (not under a profiler
+    # already: the counter passes start one of their own)
+    under_profiler = any(k.startswith("ROCPROF") for k in os.environ)
+    if kernel == v.KERNEL_I8 and args.traffic == "live" and not under_profiler:
+        progress("traffic: two rocprofv3 counter passes")
+        t = live_traffic(roof["kernel"], n, measured_variant)
+        if t is not None:
+            roof["traffic_summary"] = roof["traffic"]
+            roof["traffic"] = t
+            roof["traffic_source"] = ("rocprofv3 FETCH_SIZE and WRITE_SIZE passes in this run (the same job, "
+                                      "20 launches, medians), scaled by a 256 MiB copy kernel in the same pass")
     if not args.no_extra and n == 8192 and kernel == v.KERNEL_I8:
         out["other_workloads"] = run_extra(args, v)
     print(json.dumps(out), flush=True)
@@ -422,6 +517,7 @@ def run_c3(args, v):
                        "peak": INT8_PEAK_TOPS, "unit": "TOP/s", "frac": achieved / INT8_PEAK_TOPS,
                        "traffic": (measured_traffic(scan_kernel, TRAFFIC_SUMMARY_BATCH)
                                    if n_pairs == 256 else None),
+                       "traffic_source": "committed rocprofv3 summary" if n_pairs == 256 else None,
                        "launch_avg_us": scan_avg * 1e6}
     out["cpu_baseline"] = {"value": cpu_pairs / cpu_t, "unit": "descriptor pairs scored/s", "cores": threads,
                            "kind": "port", "sample": f"{len(sample)} of the pairs ({cpu_pairs} descriptor pairs), "
@@ -627,6 +723,7 @@ def run_c5(args, v):
                        "peak": INT8_PEAK_TOPS,
                        "unit": "TOP/s", "frac": achieved / INT8_PEAK_TOPS,
                        "traffic": measured_traffic(scan_kernel, TRAFFIC_SUMMARY_65536),
+                       "traffic_source": "committed rocprofv3 summary",
                        "launch_avg_us": scan_avg * 1e6}
     out["learned_top32"] = {"ms_per_pair": dt_l * 1e3, "effective_pairs_per_s": n * n / dt_l,
                             "recall": hit / max(len(e), 1), "precision": hit / max(len(g), 1),
