@@ -69,21 +69,41 @@ def measured_traffic(kernel_name, summary=TRAFFIC_SUMMARY):
 
 
 def traffic_child(args, v):
-    """Counter-pass child of live_traffic: the default pair's job with the scan
-    variant the parent ran, 20 times, then the 256 MiB copy kernel whose
-    counter readings scale the scan's."""
-    n = args.n
-    d = v.synth_pair(v.synth_seed(1, 0), n, n, 0.4, 0.05)
-    m = v.SIFTMatching(0)
-    m.enableRoundingWarning(False)
-    m.allocate(n, n)
-    m.setLeftFeatures(n, d["L"])
-    m.setRightFeatures(n, d["R"])
-    m.setKernel(v.KERNEL_I8)
-    m.setScanVariant(args.scan_variant)
-    for _ in range(20):
-        m.findAsync(0.8, 0.8)
-    m.synchronize()
+    """Counter-pass child of live_traffic: the workload's job with the scan
+    variant the parent ran (c2: the 8192^2 pair, 20 jobs; c3: the 256-pair
+    batch, 5 jobs; c5: the 65536^2 pair, 5 jobs), then the 256 MiB copy
+    kernel whose counter readings scale the scan's."""
+    if args.workload == "c3":
+        n_pairs = args.pairs or 256
+        rng = np.random.default_rng(2024)
+        counts = rng.integers(500, 12001, size=(n_pairs, 2))
+        images = []
+        for i in range(n_pairs):
+            d = v.synth_pair(v.synth_seed(3, i), int(counts[i, 0]), int(counts[i, 1]), 0.4, 0.05)
+            images += [d["L"], d["R"]]
+        m = v.SIFTMatching(0)
+        m.enableRoundingWarning(False)
+        m.setImages(images)
+        m.setPairs([(2 * i, 2 * i + 1) for i in range(n_pairs)])
+        m.batchSetGraph(False)
+        m.batchSetVariant(args.scan_variant)
+        for _ in range(5):
+            m.batchMatchAsync(0.8, 0.8)
+        m.batchFetch()
+    else:
+        n, seed, rho, sigma, jobs = ((65536, v.synth_seed(5, 0), 0.3, 0.08, 5) if args.workload == "c5"
+                                     else (args.n, v.synth_seed(1, 0), 0.4, 0.05, 20))
+        d = v.synth_pair(seed, n, n, rho, sigma)
+        m = v.SIFTMatching(0)
+        m.enableRoundingWarning(False)
+        m.allocate(n, n)
+        m.setLeftFeatures(n, d["L"])
+        m.setRightFeatures(n, d["R"])
+        m.setKernel(v.KERNEL_I8)
+        m.setScanVariant(args.scan_variant)
+        for _ in range(jobs):
+            m.findAsync(0.8, 0.8)
+        m.synchronize()
     m.close()
     copy_probe(256)
 
@@ -97,12 +117,15 @@ def _short_kernel(name):
     return name.split("(")[0]
 
 
-def live_traffic(kernel_name, n, variant, timeout_s=150):
+def live_traffic(kernel_name, child_args, timeout_s=150):
     """HBM bytes per launch of `kernel_name` measured now: two rocprofv3
-    counter passes (FETCH_SIZE, WRITE_SIZE) over a child process that repeats
-    this pair's job with the same scan variant and then runs the 256 MiB copy
-    kernel, each counter scaled by what it reads for that copy.  None if
-    rocprofv3 or the child is unavailable or fails."""
+    counter passes (FETCH_SIZE, WRITE_SIZE) over a child process (bench.py
+    --traffic-child + child_args) that repeats the workload's job with the
+    same scan variant and then runs the 256 MiB copy kernel, each counter
+    scaled by what it reads for that copy.  None if rocprofv3 or the child is
+    unavailable or fails, or when this process already runs under a profiler."""
+    if any(k.startswith("ROCPROF") for k in os.environ):
+        return None
     import csv
     import glob
     import shutil
@@ -117,7 +140,7 @@ def live_traffic(kernel_name, n, variant, timeout_s=150):
         try:
             cmd = ["rocprofv3", "--pmc", counter, "--kernel-include-regex", "k_i8_scan|k_copy_probe",
                    "--output-format", "csv", "-d", tmp, "-o", "t", "--", sys.executable,
-                   os.path.join(ROOT, "bench.py"), "--traffic-child", "--n", str(n), "--scan-variant", str(variant)]
+                   os.path.join(ROOT, "bench.py"), "--traffic-child"] + [str(x) for x in child_args]
             r = subprocess.run(cmd, cwd="/tmp", env=dict(os.environ, TMPDIR="/tmp"), timeout=timeout_s,
                                stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
             files = glob.glob(os.path.join(tmp, "**", "*counter_collection.csv"), recursive=True)
@@ -140,6 +163,19 @@ def live_traffic(kernel_name, n, variant, timeout_s=150):
         finally:
             shutil.rmtree(tmp, ignore_errors=True)
     return per["FETCH_SIZE"] + per["WRITE_SIZE"]
+
+
+def _live_traffic_into(roof, kernel_name, child_args):
+    """Replaces roof["traffic"] (the committed summary's figure, kept as
+    traffic_summary) with the one measured in this run, when that works."""
+    progress("traffic: two rocprofv3 counter passes")
+    t = live_traffic(kernel_name, child_args)
+    if t is not None:
+        roof["traffic_summary"] = roof["traffic"]
+        roof["traffic"] = t
+        roof["traffic_source"] = ("rocprofv3 FETCH_SIZE and WRITE_SIZE passes in this run (the same job with the "
+                                  "same scan variant, medians over its launches), scaled by a 256 MiB copy kernel "
+                                  "in the same pass")
 
 
 def copy_probe(mib, reps=5):
@@ -362,17 +398,8 @@ def run_c2(args, v):
     m.close()
     if args.copy_probe_mib > 0:
         copy_probe(args.copy_probe_mib)
-    # HBM traffic of this run's scan kernel, measured now (not under a profiler
-    # already: the counter passes start one of their own)
-    under_profiler = any(k.startswith("ROCPROF") for k in os.environ)
-    if kernel == v.KERNEL_I8 and args.traffic == "live" and not under_profiler:
-        progress("traffic: two rocprofv3 counter passes")
-        t = live_traffic(roof["kernel"], n, measured_variant)
-        if t is not None:
-            roof["traffic_summary"] = roof["traffic"]
-            roof["traffic"] = t
-            roof["traffic_source"] = ("rocprofv3 FETCH_SIZE and WRITE_SIZE passes in this run (the same job, "
-                                      "20 launches, medians), scaled by a 256 MiB copy kernel in the same pass")
+    if kernel == v.KERNEL_I8 and args.traffic == "live":
+        _live_traffic_into(roof, roof["kernel"], ["--workload", "c2", "--n", n, "--scan-variant", measured_variant])
     if not args.no_extra and n == 8192 and kernel == v.KERNEL_I8:
         out["other_workloads"] = run_extra(args, v)
     print(json.dumps(out), flush=True)
@@ -519,6 +546,9 @@ def run_c3(args, v):
                                    if n_pairs == 256 else None),
                        "traffic_source": "committed rocprofv3 summary" if n_pairs == 256 else None,
                        "launch_avg_us": scan_avg * 1e6}
+    if args.traffic == "live":
+        _live_traffic_into(out["roofline"], scan_kernel,
+                           ["--workload", "c3", "--pairs", n_pairs, "--scan-variant", best])
     out["cpu_baseline"] = {"value": cpu_pairs / cpu_t, "unit": "descriptor pairs scored/s", "cores": threads,
                            "kind": "port", "sample": f"{len(sample)} of the pairs ({cpu_pairs} descriptor pairs), "
                                                      "oracle blocked rule; their tables equal the GPU's"}
@@ -725,6 +755,8 @@ def run_c5(args, v):
                        "traffic": measured_traffic(scan_kernel, TRAFFIC_SUMMARY_65536),
                        "traffic_source": "committed rocprofv3 summary",
                        "launch_avg_us": scan_avg * 1e6}
+    if args.traffic == "live":
+        _live_traffic_into(out["roofline"], scan_kernel, ["--workload", "c5", "--scan-variant", scan_var])
     out["learned_top32"] = {"ms_per_pair": dt_l * 1e3, "effective_pairs_per_s": n * n / dt_l,
                             "recall": hit / max(len(e), 1), "precision": hit / max(len(g), 1),
                             "fallback_fraction_left": st["flagged_left"] / n,
