@@ -117,6 +117,25 @@ def _short_kernel(name):
     return name.split("(")[0]
 
 
+def scaled_counter_bytes(csv_path, kernel_name):
+    """From one rocprofv3 counter pass (counter_collection.csv): the median
+    reading of `kernel_name`'s dispatches, in bytes, scaled by the median
+    reading of the 256 MiB copy kernel of the same pass (which reads and
+    writes exactly 262144 KiB per launch).  None if either is missing."""
+    import csv
+    scan, copy = [], []
+    with open(csv_path) as f:
+        for row in csv.DictReader(f):
+            k = _short_kernel(row["Kernel_Name"])
+            if k == kernel_name:
+                scan.append(float(row["Counter_Value"]))
+            elif k == "k_copy_probe":
+                copy.append(float(row["Counter_Value"]))
+    if not scan or not copy or statistics.median(copy) <= 0:
+        return None
+    return statistics.median(scan) * 262144.0 / statistics.median(copy) * 1024.0
+
+
 def live_traffic(kernel_name, child_args, timeout_s=150):
     """HBM bytes per launch of `kernel_name` measured now: two rocprofv3
     counter passes (FETCH_SIZE, WRITE_SIZE) over a child process (bench.py
@@ -126,7 +145,6 @@ def live_traffic(kernel_name, child_args, timeout_s=150):
     unavailable or fails, or when this process already runs under a profiler."""
     if any(k.startswith("ROCPROF") for k in os.environ):
         return None
-    import csv
     import glob
     import shutil
     import subprocess
@@ -146,18 +164,10 @@ def live_traffic(kernel_name, child_args, timeout_s=150):
             files = glob.glob(os.path.join(tmp, "**", "*counter_collection.csv"), recursive=True)
             if r.returncode != 0 or not files:
                 return None
-            scan, copy = [], []
-            with open(files[0]) as f:
-                for row in csv.DictReader(f):
-                    k = _short_kernel(row["Kernel_Name"])
-                    if k == kernel_name:
-                        scan.append(float(row["Counter_Value"]))
-                    elif k == "k_copy_probe":
-                        copy.append(float(row["Counter_Value"]))
-            if not scan or not copy or statistics.median(copy) <= 0:
+            b = scaled_counter_bytes(files[0], kernel_name)
+            if b is None:
                 return None
-            # the copy kernel reads and writes exactly 256 MiB = 262144 KiB per launch
-            per[counter] = statistics.median(scan) * 262144.0 / statistics.median(copy) * 1024.0
+            per[counter] = b
         except (OSError, subprocess.SubprocessError, KeyError, ValueError):
             return None
         finally:
