@@ -88,7 +88,7 @@ def traffic_child(args, v):
         m.batchSetGraph(False)
         m.batchSetVariant(args.scan_variant)
         for _ in range(5):
-            m.batchMatchAsync(0.8, 0.8)
+            m.batchMatchAsync(args.min_score, 0.8)
         m.batchFetch()
     else:
         n, seed, rho, sigma, jobs = ((65536, v.synth_seed(5, 0), 0.3, 0.08, 5) if args.workload == "c5"
@@ -219,6 +219,9 @@ def main():
                          "benchmark pair; mfma-ceiling: the int8 scan's step loop alone (no global loads, barriers "
                          "or survivors) on the benchmark pair's descriptor bytes and on other operand data")
     ap.add_argument("--pairs", type=int, default=0, help="number of pairs for c3 / c4 (0: the workload's own)")
+    ap.add_argument("--min-score", type=float, default=0.8,
+                    help="c3: the rule's minScore (above 1 no score survives the scan's filter: its time without "
+                         "survivors)")
     ap.add_argument("--scan-variant", type=int, default=-1,
                     help="single-pair int8 scan variant (-1: the library measures the fastest)")
     ap.add_argument("--profile-variants", action="store_true",
@@ -480,6 +483,7 @@ def run_c3(args, v):
     from oracle import oracle as O
 
     n_pairs = args.pairs or 256
+    ms = args.min_score
     rng = np.random.default_rng(2024)
     counts = rng.integers(500, 12001, size=(n_pairs, 2))
     images, fimages = [], []
@@ -493,34 +497,34 @@ def run_c3(args, v):
     c32 = m.setImages(images)
     m.setPairs([(2 * i, 2 * i + 1) for i in range(n_pairs)])
     pairs_scored = int(sum(c32[2 * i] * c32[2 * i + 1] for i in range(n_pairs)))
-    best, tune_ms = m.batchAutotune(0.8, 0.8, reps=2)
+    best, tune_ms = m.batchAutotune(ms, 0.8, reps=2)
     m.batchSetVariant(best)
     # scan kernel time (events bound to the scan dispatch, eager jobs)
     m.batchSetGraph(False)
     for _ in range(3):
-        m.batchMatchAsync(0.8, 0.8)
+        m.batchMatchAsync(ms, 0.8)
     m.batchFetch()
     # median over jobs profiled one at a time: one slow job (a clock ramp)
     # does not move it
     per_job = []
     for _ in range(15):
         m.profileEnable(True)
-        m.batchMatchAsync(0.8, 0.8)
+        m.batchMatchAsync(ms, 0.8)
         m.batchFetch()
         s_ms, _, nj = m.profileRead()
         per_job.append(s_ms / max(nj, 1))
     m.profileEnable(False)
     scan_ms, jobs = float(np.median(per_job)), 1
     m.batchSetGraph(True)
-    tabs = m.batchMatch(0.8, 0.8)
+    tabs = m.batchMatch(ms, 0.8)
     warm = max(args.warmup // 20, 3)
     for _ in range(warm):
-        m.batchMatchAsync(0.8, 0.8)
+        m.batchMatchAsync(ms, 0.8)
     m.batchFetch()
     steps = max(args.steps // 100, 10)
     t0 = time.perf_counter()
     for _ in range(steps):
-        m.batchMatchAsync(0.8, 0.8)
+        m.batchMatchAsync(ms, 0.8)
     tabs2 = m.batchFetch()
     wall = (time.perf_counter() - t0) / steps
     assert all(np.array_equal(a, b) for a, b in zip(tabs, tabs2))
@@ -531,7 +535,7 @@ def run_c3(args, v):
     for i in sample:
         L, R = images[2 * i][:c32[2 * i]], images[2 * i + 1][:c32[2 * i + 1]]
         t = time.perf_counter()
-        _, _, tab, used = O.find_u8_blocked(L, R, 0.8, 0.8, threads=threads)
+        _, _, tab, used = O.find_u8_blocked(L, R, ms, 0.8, threads=threads)
         cpu_t += time.perf_counter() - t
         cpu_pairs += len(L) * len(R)
         agree &= np.array_equal(tab, tabs[i])
@@ -543,7 +547,7 @@ def run_c3(args, v):
     w, tm, tn = v.scan_variants()[best]
     out = _common("descriptor pairs scored/s", pairs_scored / wall, "descriptor pairs/s", steps, warm, wall, "int8",
                   {"workload": f"{n_pairs} image pairs, feature counts uniform in [500, 12000], 40% noisy copies "
-                               "(sigma 0.05), minScore 0.8, minRatio 0.8, mutual check; one batched job",
+                               f"(sigma 0.05), minScore {ms}, minRatio 0.8, mutual check; one batched job",
                    "pairs": n_pairs, "descriptor_pairs_per_job": pairs_scored, "variant": best,
                    "tile": [tm, tn], "waves": w, "graph": True})
     out["image_pairs_per_s"] = n_pairs / wall
@@ -558,7 +562,7 @@ def run_c3(args, v):
                        "launch_avg_us": scan_avg * 1e6}
     if args.traffic == "live":
         _live_traffic_into(out["roofline"], scan_kernel,
-                           ["--workload", "c3", "--pairs", n_pairs, "--scan-variant", best])
+                           ["--workload", "c3", "--pairs", n_pairs, "--scan-variant", best, "--min-score", ms])
     out["cpu_baseline"] = {"value": cpu_pairs / cpu_t, "unit": "descriptor pairs scored/s", "cores": threads,
                            "kind": "port", "sample": f"{len(sample)} of the pairs ({cpu_pairs} descriptor pairs), "
                                                      "oracle blocked rule; their tables equal the GPU's"}
@@ -571,17 +575,17 @@ def run_c3(args, v):
     fm.enableRoundingWarning(False)
     fc32 = fm.setImages(fimages)
     fm.setPairs([(2 * i, 2 * i + 1) for i in range(n_pairs)])
-    fbest, _ = fm.batchAutotune(0.8, 0.8, reps=2)
+    fbest, _ = fm.batchAutotune(ms, 0.8, reps=2)
     fm.batchSetVariant(fbest)
     fm.batchSetGraph(True)
-    ftabs = fm.batchMatch(0.8, 0.8)
+    ftabs = fm.batchMatch(ms, 0.8)
     for _ in range(3):
-        fm.batchMatchAsync(0.8, 0.8)
+        fm.batchMatchAsync(ms, 0.8)
     fm.batchFetch()
     t0 = time.perf_counter()
     fsteps = max(steps // 2, 5)
     for _ in range(fsteps):
-        fm.batchMatchAsync(0.8, 0.8)
+        fm.batchMatchAsync(ms, 0.8)
     fm.batchFetch()
     f_wall = (time.perf_counter() - t0) / fsteps
     fm.close()
@@ -592,15 +596,15 @@ def run_c3(args, v):
     sm.setKernel(v.KERNEL_BF16)
     for i in sample:
         Lf, Rf = fimages[2 * i][:fc32[2 * i]], fimages[2 * i + 1][:fc32[2 * i + 1]]
-        exact_f32 &= np.array_equal(ftabs[i], O.find_f32(Lf, Rf, 0.8, 0.8)[2])
+        exact_f32 &= np.array_equal(ftabs[i], O.find_f32(Lf, Rf, ms, 0.8)[2])
         got = O.table_to_pairs(ftabs[i])
-        l64 = set(O.table_to_pairs(O.find_f64(Lf, Rf, 0.8, 0.8)[2]))
+        l64 = set(O.table_to_pairs(O.find_f64(Lf, Rf, ms, 0.8)[2]))
         disagree += len(set(got) ^ l64)
         matches += len(l64)
         # the same rule's rescored scores (single-pair path) against FP64
         sm.setLeftFeatures(len(Lf), Lf)
         sm.setRightFeatures(len(Rf), Rf)
-        sm.findPutativeMatches(0.8, 0.8)
+        sm.findPutativeMatches(ms, 0.8)
         bsc, _, arg = sm.leftTop2()
         have = arg >= 0
         if have.any():
